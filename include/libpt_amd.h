@@ -1,0 +1,107 @@
+/*
+ * Extension API of the MI355X perturbative-triples engine.  These entry
+ * points have no counterpart in the reference libpt (src/pt.h); they add
+ * what the drop-in API of pt.h cannot express:
+ *   - a choice of arithmetic form (FP64, FP32, split BF16 / FP16),
+ *   - inputs that already live in GPU memory (no host copy),
+ *   - running a subset of the ijk work list and reading per-ijk energies,
+ *   - the deterministic synthetic-input generator used by tests and bench.
+ *
+ * Array arguments are always passed in pt.h order:
+ *   (T)  : d_ov, f_ov, t1, t2, i_oovo, i_oovv, i_ovvv
+ *   (fT) : d_ov, f2_ov, l1, t2, l2, i_oovv, i2_t2f2_oovo, i3_ovvv,
+ *          i6_oovo, i7_ovvv
+ * with the layouts of pt.h.  Element type: double for the FP64, BF16X3
+ * and F16X3 forms, float for the FP32 form.
+ */
+#ifndef LIBPT_AMD_H
+#define LIBPT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum libpt_amd_form {
+	LIBPT_AMD_FP64 = 0,   /* FP64 MFMA (libpt_rpt & co.) */
+	LIBPT_AMD_FP32 = 1,   /* FP32 MFMA (libpt_*_mp) */
+	LIBPT_AMD_BF16X3 = 2, /* hi+lo BF16 products, FP32 accumulation */
+	LIBPT_AMD_F16X3 = 3   /* hi+lo FP16 products with power-of-two scaling */
+};
+
+enum libpt_amd_call {
+	LIBPT_AMD_RPT = 0, LIBPT_AMD_UPT = 1, LIBPT_AMD_RFT = 2, LIBPT_AMD_UFT = 3
+};
+
+struct libpt_amd_stats {
+	double wall_s;    /* whole call */
+	double h2d_s;     /* host -> device copies */
+	double prep_s;    /* operand preparation (split, scaling) */
+	double kernel_s;  /* tile kernels and reductions */
+	double flop;      /* reference-formulation GEMM flops of the call */
+	double scale[4];  /* operand scales (F16X3): amp/int of W sets 0, 1 */
+};
+
+/* One call with host arrays in a chosen form.  Returns the energy with
+ * the same meaning as the pt.h function of `call`; NaN on error. */
+double libpt_amd_energy(int call, int form, size_t oa, size_t va, size_t ob,
+    size_t vb, const void *const *arrays, struct libpt_amd_stats *stats);
+
+/* ---- plans over device-resident inputs ---- */
+typedef struct libpt_amd_plan libpt_amd_plan;
+
+/* dev_arrays: device pointers in pt.h order; the plan keeps pointers
+ * (not copies) for FP64/FP32 and allocates split operands for the split
+ * forms.  NULL on error. */
+libpt_amd_plan *libpt_amd_plan_create(int call, int form, size_t oa,
+    size_t va, size_t ob, size_t vb, const void *const *dev_arrays);
+void libpt_amd_plan_destroy(libpt_amd_plan *plan);
+
+/* spin cases of the call (rpt/rft: 2, upt/uft: 4) and their sizes */
+int libpt_amd_plan_ncases(const libpt_amd_plan *plan);
+int64_t libpt_amd_plan_nijk(const libpt_amd_plan *plan, int sc);
+int64_t libpt_amd_plan_items(const libpt_amd_plan *plan, int sc);
+double libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *plan, int sc);
+/* multiplier of spin case sc in the call's energy (2 for restricted) */
+double libpt_amd_plan_factor(const libpt_amd_plan *plan, int sc);
+
+/* enqueue ijk items [ijk_begin, ijk_end) of spin case sc on the plan's
+ * stream; 0 on success */
+int libpt_amd_plan_run(libpt_amd_plan *plan, int sc, int64_t ijk_begin,
+    int64_t ijk_end);
+int libpt_amd_plan_sync(libpt_amd_plan *plan);
+/* energy of the call from the items run so far (all items: the full
+ * energy); synchronises */
+double libpt_amd_plan_energy(libpt_amd_plan *plan);
+/* per-ijk energies of spin case sc (without the factor), n_ijk doubles */
+int libpt_amd_plan_ijk_energies(libpt_amd_plan *plan, int sc, double *out);
+/* the ijk triples of spin case sc, 3 * n_ijk ints, in work-list order */
+int libpt_amd_plan_ijk_list(const libpt_amd_plan *plan, int sc, int32_t *out);
+
+/* GPU-timed window on the plan's stream: begin/end bracket enqueued work */
+int libpt_amd_plan_timer_begin(libpt_amd_plan *plan);
+double libpt_amd_plan_timer_end_ms(libpt_amd_plan *plan);
+
+/* ---- device memory and synthetic inputs ---- */
+void *libpt_amd_dev_alloc(size_t bytes);
+void libpt_amd_dev_free(void *ptr);
+int libpt_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);
+
+/* Fill one pt.h input array with the deterministic generator (see
+ * libpt_amd/csrc/synth.h).  spec points to a struct pt_synth_spec;
+ * tensor is a pt_synth_tensor id; on_device selects device or host
+ * memory; fp32 writes float instead of double. */
+int libpt_amd_synth(const void *spec, int tensor, void *dst, int on_device,
+    int fp32);
+int64_t libpt_amd_synth_size(const void *spec, int tensor);
+
+/* message of the last failed call of this API */
+const char *libpt_amd_last_error(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* LIBPT_AMD_H */

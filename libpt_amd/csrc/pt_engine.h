@@ -1,0 +1,96 @@
+/*
+ * Internal interface between the host engine and the device kernels of
+ * libpt_amd (not part of the public C API).
+ *
+ * A spin case of one system is a list of *terms*.  Each term adds
+ *     sign * sum_kappa S[s][kappa] * Q[kappa][u][w]
+ * to the connected triples W[a][b][c] of one ijk item, where the single
+ * index s and the pair (u, w) are the output indices a, b, c in some role
+ * (role = which of a/b/c the S operand carries; (u, w) are the other two in
+ * a, b, c order).  S and Q are strided slices of the input tensors whose
+ * base offsets are linear in the occupied indices (i, j, k).
+ *
+ * The terms of every spin case of libpt (src/pt.c:236-290, :361-432,
+ * :588-696, :773-915) are written in this form by pt_terms.cpp.
+ */
+#ifndef LIBPT_AMD_PT_ENGINE_H
+#define LIBPT_AMD_PT_ENGINE_H
+
+#include <stdint.h>
+
+enum pt_form {
+	PT_FORM_FP64 = 0,   /* FP64 operands, v_mfma_f64_16x16x4_f64 */
+	PT_FORM_FP32 = 1,   /* FP32 operands, v_mfma_f32_16x16x4_f32 (_mp) */
+	PT_FORM_BF16X3 = 2, /* hi+lo BF16 pairs, 3 v_mfma_f32_16x16x32_bf16 */
+	PT_FORM_F16X3 = 3,  /* hi+lo FP16 pairs (scaled), 3 x 16x16x32_f16 */
+	PT_NFORM = 4
+};
+
+enum pt_kind { PT_AAA = 0, PT_AAB = 1 };
+enum pt_role { PT_ROLE_A = 0, PT_ROLE_B = 1, PT_ROLE_C = 2 };
+
+#define PT_MAX_TERMS 40
+#define PT_MAX_BUFS 20
+
+/* workgroup tile: 32 a x 32 b x 16 c, four waves of 16 x 16 x 16 */
+#define PT_TA 32
+#define PT_TB 32
+#define PT_TC 16
+#define PT_THREADS 256
+
+struct pt_term {
+	int64_t s_off[3];  /* S base offset = s_off . (i, j, k) */
+	int64_t q_off[3];  /* Q base offset = q_off . (i, j, k) */
+	int64_t s_st;      /* stride of the single index s (kappa stride 1) */
+	int64_t q_k;       /* stride of kappa in Q */
+	int64_t q_st[3];   /* strides of a, b, c in Q (the S role's unused) */
+	int32_t s_buf, q_buf;
+	int32_t role, set, sign, K;
+	int32_t packed;    /* Q pair (u > w) is a packed triangle u(u-1)/2 + w */
+	int32_t ufast;     /* u is the contiguous Q index (else w) */
+};
+
+/* epilogue inputs (element type: double, or float for PT_FORM_FP32) */
+struct pt_epi {
+	const void *d1, *d2;  /* d_ov of the pair spin / odd spin */
+	const void *u1, *u2;  /* t1 (T) or l1 (fT) */
+	const void *f1, *f2;  /* f_ov (T) or f2_ov (fT) */
+	const void *GA, *GB;  /* i_oovv same-spin / mixed blocks */
+	const void *ZA, *ZB;  /* t2 (T) or l2 (fT) same-spin / mixed blocks */
+};
+
+struct pt_launch {
+	const void *buf[PT_MAX_BUFS];
+	const struct pt_term *terms; /* device array: phase-1 terms, then phase 2 */
+	int32_t nterm1, nterm2;
+	int32_t kind, nset;
+	int64_t oa, va, ob, vb;      /* aaa: ob = oa, vb = va */
+	int32_t na, nc;              /* extents of a (= b) and c */
+	int32_t nA, nC;              /* 32-blocks over a/b, 16-blocks over c */
+	const int32_t *ijk;          /* [n_ijk][3] */
+	const int32_t *tiles;        /* [n_tiles][2]: (A block, B block), A >= B */
+	int32_t n_tiles, pad_;
+	int64_t item0, n_chunk;      /* items [item0, item0 + n_chunk) */
+	double *partial;             /* energy of each item */
+	double inv_scale[2];         /* per W set: 1 / (operand scale product) */
+	struct pt_epi epi;
+};
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* kernel launchers (pt_kernel.hip); return hipError_t as int */
+int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
+int pt_launch_reduce(const double *x, int64_t n, double *out, void *stream);
+int pt_launch_segsum(const double *x, int64_t nseg, int64_t seglen,
+    double *out, void *stream);
+int pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
+    double scale, void *stream);
+int pt_launch_absmax(const double *src, int64_t n, double *out, void *stream);
+int pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n,
+    int fp32, void *stream);
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBPT_AMD_PT_ENGINE_H */

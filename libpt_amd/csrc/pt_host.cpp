@@ -1,0 +1,954 @@
+/*
+ * Host side of the MI355X perturbative-triples engine: the pt.h drop-in
+ * entry points (reference src/pt.h, src/pt.c:454-535, 936-1050,
+ * src/ptmp.c same lines, src/misc.c) and the extension API of
+ * libpt_amd.h.
+ *
+ * One call = one plan: stage inputs in HBM, build the work list of every
+ * spin case (ijk items x (A, B) tiles, ijk-major), launch the tile kernel
+ * over it, reduce the per-item energies in a fixed order, and combine the
+ * spin cases with the reference's factors.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/libpt_amd.h"
+#include "../../include/pt.h"
+#include "pt_engine.h"
+#include "pt_terms.h"
+#include "synth.h"
+
+namespace {
+
+void *(*g_malloc)(size_t) = malloc;
+void (*g_free)(void *) = free;
+char g_err[512];
+bool g_dev_ready = false;
+
+void
+set_err(const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof(g_err), fmt, ap);
+	va_end(ap);
+}
+
+double
+now_s()
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+#define HCHK(call, ret)                                                      \
+	do {                                                                 \
+		hipError_t e_ = (call);                                      \
+		if (e_ != hipSuccess) {                                      \
+			set_err("%s failed: %s", #call, hipGetErrorString(e_)); \
+			return ret;                                          \
+		}                                                            \
+	} while (0)
+
+bool
+dev_init()
+{
+	if (g_dev_ready)
+		return true;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+		set_err("no HIP device available");
+		return false;
+	}
+	const char *e = getenv("LIBPT_AMD_DEVICE");
+	int dev = e ? atoi(e) : 0;
+	if (dev < 0 || dev >= n) {
+		set_err("LIBPT_AMD_DEVICE=%d out of range (%d devices)", dev, n);
+		return false;
+	}
+	HCHK(hipSetDevice(dev), false);
+	g_dev_ready = true;
+	return true;
+}
+
+/* host scratch through the libpt_set_malloc hooks (src/misc.c:22-47) */
+template <class T>
+struct HostBuf {
+	T *p = nullptr;
+	size_t n = 0;
+	bool alloc(size_t count)
+	{
+		p = (T *)g_malloc((count ? count : 1) * sizeof(T));
+		n = count;
+		if (!p)
+			set_err("libpt malloc failed (%zu bytes)", count * sizeof(T));
+		return p != nullptr;
+	}
+	~HostBuf()
+	{
+		if (p)
+			g_free(p);
+	}
+};
+
+/* element counts of the API arrays in pt.h order */
+int
+api_sizes(int call, int64_t oa, int64_t va, int64_t ob, int64_t vb,
+    int64_t n[10])
+{
+	const bool ft = call == LIBPT_AMD_RFT || call == LIBPT_AMD_UFT;
+	const bool r = call == LIBPT_AMD_RPT || call == LIBPT_AMD_RFT;
+	int64_t ov, oovv, oovo, ovvv;
+	if (r) {
+		ov = oa * va;
+		oovv = 2 * oa * oa * va * va;
+		oovo = 2 * oa * oa * va * oa;
+		ovvv = oa * va * (va * (va - 1) / 2) + oa * va * va * va;
+	} else {
+		ov = oa * va + ob * vb;
+		oovv = oa * oa * va * va + 2 * oa * ob * va * vb + ob * ob * vb * vb;
+		oovo = oa * oa * va * oa + oa * ob * va * ob + ob * ob * vb * ob +
+		    ob * oa * vb * oa;
+		ovvv = oa * va * (va * (va - 1) / 2) + oa * vb * va * vb +
+		    ob * vb * (vb * (vb - 1) / 2) + ob * va * vb * va;
+	}
+	if (!ft) {
+		int64_t s[7] = {ov, ov, ov, oovv, oovo, oovv, ovvv};
+		memcpy(n, s, sizeof(s));
+		return 7;
+	}
+	int64_t s[10] = {ov, ov, ov, oovv, oovv, oovv, oovo, ovvv, oovo, ovvv};
+	memcpy(n, s, sizeof(s));
+	return 10;
+}
+
+double
+flop_same(double o, double v)
+{
+	return 3.0 * v * v * v * (v - 1) + 6.0 * o * v * v * v;
+}
+
+double
+flop_mixed(double oa, double va, double ob, double vb)
+{
+	return 2 * va * va * va * vb + 4 * va * va * vb * vb +
+	    2 * vb * va * va * (va - 1) + 6 * oa * va * va * vb +
+	    4 * ob * va * va * vb;
+}
+
+struct Case {
+	pt_launch L;
+	std::vector<pt_term> terms;
+	pt_term *d_terms = nullptr;
+	int32_t *d_ijk = nullptr, *d_tiles = nullptr;
+	double *d_partial = nullptr;
+	int64_t n_ijk = 0;
+	int n_tiles = 0;
+	double factor = 1.0, flop_per_ijk = 0.0;
+};
+
+/* block pointers of one 4-index family: [aaaa, abab, bbbb, baba] */
+struct Blocks {
+	const char *p[4];
+};
+
+} /* namespace */
+
+struct libpt_amd_plan {
+	int call = 0, form = 0;
+	int64_t oa = 0, va = 0, ob = 0, vb = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	std::vector<Case> cases;
+	std::vector<void *> owned;
+	double *d_out = nullptr;
+	double scale[4] = {1, 1, 1, 1};
+	double prep_s = 0;
+};
+
+namespace {
+
+void
+plan_free(libpt_amd_plan *pl)
+{
+	if (!pl)
+		return;
+	if (pl->stream)
+		(void)hipStreamSynchronize(pl->stream);
+	for (Case &c : pl->cases) {
+		(void)hipFree(c.d_terms);
+		(void)hipFree(c.d_ijk);
+		(void)hipFree(c.d_tiles);
+		(void)hipFree(c.d_partial);
+	}
+	for (void *p : pl->owned)
+		(void)hipFree(p);
+	(void)hipFree(pl->d_out);
+	if (pl->ev0)
+		(void)hipEventDestroy(pl->ev0);
+	if (pl->ev1)
+		(void)hipEventDestroy(pl->ev1);
+	if (pl->stream)
+		(void)hipStreamDestroy(pl->stream);
+	delete pl;
+}
+
+bool
+dev_alloc(void **p, size_t bytes)
+{
+	hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+	if (e != hipSuccess) {
+		set_err("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+		*p = nullptr;
+		return false;
+	}
+	return true;
+}
+
+/* work list of one spin case: ijk items and (A >= B) tiles */
+bool
+case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na)
+{
+	int64_t nijk = same ? o1 * (o1 - 1) * (o1 - 2) / 6 : o1 * (o1 - 1) / 2 * o2;
+	HostBuf<int32_t> ijk;
+	if (!ijk.alloc(3 * (size_t)(nijk > 0 ? nijk : 1)))
+		return false;
+	int64_t it = 0;
+	for (int64_t i = 0; i < o1; ++i)
+		for (int64_t j = i + 1; j < o1; ++j)
+			for (int64_t k = same ? j + 1 : 0; k < (same ? o1 : o2); ++k) {
+				ijk.p[3 * it] = (int32_t)i;
+				ijk.p[3 * it + 1] = (int32_t)j;
+				ijk.p[3 * it + 2] = (int32_t)k;
+				++it;
+			}
+	int nA = (int)((na + PT_TA - 1) / PT_TA);
+	HostBuf<int32_t> tiles;
+	if (!tiles.alloc((size_t)nA * (nA + 1)))
+		return false;
+	int nt = 0;
+	for (int A = 0; A < nA; ++A)
+		for (int B = 0; B <= A; ++B) {
+			tiles.p[2 * nt] = A;
+			tiles.p[2 * nt + 1] = B;
+			++nt;
+		}
+	c.n_ijk = nijk;
+	c.n_tiles = nt;
+	if (!dev_alloc((void **)&c.d_ijk, 3 * sizeof(int32_t) * (size_t)nijk) ||
+	    !dev_alloc((void **)&c.d_tiles, 2 * sizeof(int32_t) * (size_t)nt) ||
+	    !dev_alloc((void **)&c.d_partial,
+	    sizeof(double) * (size_t)(nijk * nt)))
+		return false;
+	HCHK(hipMemcpy(c.d_ijk, ijk.p, 3 * sizeof(int32_t) * (size_t)nijk,
+	    hipMemcpyHostToDevice), false);
+	HCHK(hipMemcpy(c.d_tiles, tiles.p, 2 * sizeof(int32_t) * (size_t)nt,
+	    hipMemcpyHostToDevice), false);
+	HCHK(hipMemset(c.d_partial, 0, sizeof(double) * (size_t)(nijk * nt)),
+	    false);
+	return true;
+}
+
+bool
+case_finish(Case &c, int kind, int nset, int64_t oa, int64_t va, int64_t ob,
+    int64_t vb)
+{
+	pt_launch &L = c.L;
+	L.kind = kind;
+	L.nset = nset;
+	L.oa = oa;
+	L.va = va;
+	L.ob = ob;
+	L.vb = vb;
+	L.na = (int32_t)va;
+	L.nc = (int32_t)vb;
+	L.nA = (int32_t)((va + PT_TA - 1) / PT_TA);
+	L.nC = (int32_t)((vb + PT_TC - 1) / PT_TC);
+	for (int s = 0; s < nset; ++s) {
+		if (kind == PT_AAA)
+			pt_terms_same(c.terms, s, oa, va);
+		else
+			pt_terms_mixed(c.terms, s, oa, va, ob, vb);
+	}
+	L.nterm1 = pt_terms_order(c.terms);
+	L.nterm2 = (int32_t)c.terms.size() - L.nterm1;
+	if (!dev_alloc((void **)&c.d_terms, sizeof(pt_term) * c.terms.size()))
+		return false;
+	HCHK(hipMemcpy(c.d_terms, c.terms.data(),
+	    sizeof(pt_term) * c.terms.size(), hipMemcpyHostToDevice), false);
+	L.terms = c.d_terms;
+	if (!case_lists(c, kind == PT_AAA, oa, ob, va))
+		return false;
+	L.ijk = c.d_ijk;
+	L.tiles = c.d_tiles;
+	L.n_tiles = c.n_tiles;
+	L.partial = c.d_partial;
+	c.flop_per_ijk = (kind == PT_AAA ? flop_same((double)oa, (double)va) :
+	    flop_mixed((double)oa, (double)va, (double)ob, (double)vb)) * nset;
+	return true;
+}
+
+/* power-of-two scale bringing max |x| of a family to about 2^14 */
+bool
+family_scale(libpt_amd_plan *pl, const double *const *arr, const int64_t *n,
+    int cnt, double *scale)
+{
+	HCHK(hipMemsetAsync(pl->d_out, 0, sizeof(double), pl->stream), false);
+	for (int q = 0; q < cnt; ++q)
+		if (n[q] > 0 && pt_launch_absmax(arr[q], n[q], pl->d_out,
+		    pl->stream) != 0) {
+			set_err("absmax launch failed");
+			return false;
+		}
+	double mx = 0;
+	HCHK(hipMemcpyAsync(&mx, pl->d_out, sizeof(double),
+	    hipMemcpyDeviceToHost, pl->stream), false);
+	HCHK(hipStreamSynchronize(pl->stream), false);
+	*scale = mx > 0 ? ldexp(1.0, (int)floor(log2(16384.0 / mx))) : 1.0;
+	return true;
+}
+
+} /* namespace */
+
+extern "C" libpt_amd_plan *
+libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
+    size_t vb_, const void *const *arr)
+{
+	if (!dev_init())
+		return nullptr;
+	if (call < 0 || call > 3 || form < 0 || form >= PT_NFORM) {
+		set_err("bad call %d or form %d", call, form);
+		return nullptr;
+	}
+	const bool ft = call == LIBPT_AMD_RFT || call == LIBPT_AMD_UFT;
+	const bool restricted = call == LIBPT_AMD_RPT || call == LIBPT_AMD_RFT;
+	int64_t oa = (int64_t)oa_, va = (int64_t)va_;
+	int64_t ob = restricted ? oa : (int64_t)ob_;
+	int64_t vb = restricted ? va : (int64_t)vb_;
+	libpt_amd_plan *pl = new libpt_amd_plan;
+	pl->call = call;
+	pl->form = form;
+	pl->oa = oa;
+	pl->va = va;
+	pl->ob = ob;
+	pl->vb = vb;
+#define PFAIL(cond)                  \
+	do {                         \
+		if (!(cond)) {       \
+			plan_free(pl); \
+			return nullptr; \
+		}                    \
+	} while (0)
+	PFAIL(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) ==
+	    hipSuccess || (set_err("hipStreamCreate failed"), false));
+	PFAIL(hipEventCreate(&pl->ev0) == hipSuccess &&
+	    hipEventCreate(&pl->ev1) == hipSuccess);
+	PFAIL(dev_alloc((void **)&pl->d_out, 4 * sizeof(double)));
+
+	int64_t n[10];
+	const int narr = api_sizes(call, oa, va, ob, vb, n);
+	(void)narr;
+	const bool fp32 = form == PT_FORM_FP32;
+	const bool split = form == PT_FORM_BF16X3 || form == PT_FORM_F16X3;
+	const size_t esz = fp32 ? 4 : 8;     /* epilogue element size */
+	const size_t osz = fp32 || split ? 4 : 8; /* operand element size */
+
+	/* operand arrays: (T) t2, oovo, ovvv; (fT) t2, l2, i2, i3, i6, i7 */
+	const int *opidx;
+	static const int op_t[3] = {3, 4, 6};
+	static const int op_f[6] = {3, 4, 6, 7, 8, 9};
+	const int nop = ft ? 6 : 3;
+	opidx = ft ? op_f : op_t;
+	const void *oparr[10];
+	for (int q = 0; q < 10; ++q)
+		oparr[q] = q < (ft ? 10 : 7) ? arr[q] : nullptr;
+	if (split) {
+		double t0 = now_s();
+		/* W sets: (T) one set {t2 | oovo, ovvv};
+		 * (fT) set 0 = L {l2 | i6, i7}, set 1 = R {t2 | i2, i3} */
+		const int nset = ft ? 2 : 1;
+		for (int s = 0; s < nset; ++s) {
+			int amp, i1, i2;
+			if (!ft) {
+				amp = 3; i1 = 4; i2 = 6;
+			} else if (s == 0) {
+				amp = 4; i1 = 8; i2 = 9;
+			} else {
+				amp = 3; i1 = 6; i2 = 7;
+			}
+			double sa = 1, si = 1;
+			if (form == PT_FORM_F16X3) {
+				const double *fa[1] = {(const double *)arr[amp]};
+				const double *fi[2] = {(const double *)arr[i1],
+				    (const double *)arr[i2]};
+				int64_t na_[1] = {n[amp]}, ni_[2] = {n[i1], n[i2]};
+				PFAIL(family_scale(pl, fa, na_, 1, &sa));
+				PFAIL(family_scale(pl, fi, ni_, 2, &si));
+			}
+			pl->scale[2 * s] = sa;
+			pl->scale[2 * s + 1] = si;
+			const int fam[3] = {amp, i1, i2};
+			const double fsc[3] = {sa, si, si};
+			for (int q = 0; q < 3; ++q) {
+				void *dst;
+				PFAIL(dev_alloc(&dst, sizeof(uint32_t) * (size_t)n[fam[q]]));
+				pl->owned.push_back(dst);
+				PFAIL(pt_launch_split((const double *)arr[fam[q]],
+				    (uint32_t *)dst, n[fam[q]],
+				    form == PT_FORM_F16X3, fsc[q], pl->stream) == 0 ||
+				    (set_err("split launch failed"), false));
+				oparr[fam[q]] = dst;
+			}
+		}
+		PFAIL(hipStreamSynchronize(pl->stream) == hipSuccess);
+		pl->prep_s = now_s() - t0;
+	}
+	(void)opidx;
+	(void)nop;
+
+	/* block pointers (element offsets as in src/pt.c:460-467, 487-512) */
+	auto blk4 = [&](const void *base, int fam, size_t el) {
+		Blocks b;
+		int64_t s[4];
+		if (fam == 0) {
+			s[0] = oa * oa * va * va; s[1] = oa * ob * va * vb;
+			s[2] = ob * ob * vb * vb;
+		} else if (fam == 1) {
+			s[0] = oa * oa * va * oa; s[1] = oa * ob * va * ob;
+			s[2] = ob * ob * vb * ob;
+		} else {
+			s[0] = oa * va * (va * (va - 1) / 2); s[1] = oa * vb * va * vb;
+			s[2] = ob * vb * (vb * (vb - 1) / 2);
+		}
+		const char *p = (const char *)base;
+		b.p[0] = p;
+		b.p[1] = p + el * (size_t)s[0];
+		b.p[2] = b.p[1] + el * (size_t)s[1];
+		b.p[3] = b.p[2] + el * (size_t)s[2];
+		if (restricted) { /* baba = abab, bbbb = aaaa */
+			b.p[2] = b.p[0];
+			b.p[3] = b.p[1];
+		}
+		return b;
+	};
+	auto ov2 = [&](const void *base, size_t el, const char **pa,
+	    const char **pb) {
+		*pa = (const char *)base;
+		*pb = restricted ? *pa : *pa + el * (size_t)(oa * va);
+	};
+
+	const char *dA, *dB, *fA, *fB, *uA, *uB;
+	ov2(arr[0], esz, &dA, &dB);
+	ov2(arr[1], esz, &fA, &fB);
+	ov2(arr[2], esz, &uA, &uB);
+	/* epilogue T2-like (Z uses t2 for (T), l2 for (fT)) and oovv */
+	Blocks zt = blk4(arr[ft ? 4 : 3], 0, esz);
+	Blocks gv = blk4(arr[ft ? 5 : 5], 0, esz);
+	/* operand families per W set: [set][T, O, V] */
+	Blocks opT[2], opO[2], opV[2];
+	if (!ft) {
+		opT[0] = blk4(oparr[3], 0, osz);
+		opO[0] = blk4(oparr[4], 1, osz);
+		opV[0] = blk4(oparr[6], 2, osz);
+	} else {
+		opT[0] = blk4(oparr[4], 0, osz); /* L: l2, i6, i7 */
+		opO[0] = blk4(oparr[8], 1, osz);
+		opV[0] = blk4(oparr[9], 2, osz);
+		opT[1] = blk4(oparr[3], 0, osz); /* R: t2, i2, i3 */
+		opO[1] = blk4(oparr[6], 1, osz);
+		opV[1] = blk4(oparr[7], 2, osz);
+	}
+	const int nset = ft ? 2 : 1;
+	const double fac = restricted ? 2.0 : 1.0;
+
+	/* spin cases: aaa(alpha), [aaa(beta)], aab(alpha pair), [aab(beta pair)]
+	 * (src/pt.c:469-473 restricted; :514-529 unrestricted) */
+	struct CaseSpec {
+		int kind, spin;
+	};
+	std::vector<CaseSpec> specs;
+	specs.push_back({PT_AAA, 0});
+	if (!restricted)
+		specs.push_back({PT_AAA, 1});
+	specs.push_back({PT_AAB, 0});
+	if (!restricted)
+		specs.push_back({PT_AAB, 1});
+
+	for (const CaseSpec &cs : specs) {
+		Case c;
+		memset(&c.L, 0, sizeof(c.L));
+		const int sp = cs.spin;
+		/* block indices of the pair spin (same) and mixed blocks */
+		const int same = sp == 0 ? 0 : 2;   /* aaaa or bbbb */
+		const int mixB = sp == 0 ? 1 : 3;   /* abab or baba */
+		const int mixC = sp == 0 ? 3 : 1;   /* baba or abab */
+		int64_t o1 = sp == 0 ? oa : ob, v1 = sp == 0 ? va : vb;
+		int64_t o2 = sp == 0 ? ob : oa, v2 = sp == 0 ? vb : va;
+		for (int s = 0; s < nset; ++s) {
+			c.L.buf[9 * s + 0] = opT[s].p[same];
+			c.L.buf[9 * s + 1] = opT[s].p[mixB];
+			c.L.buf[9 * s + 2] = opT[s].p[mixC];
+			c.L.buf[9 * s + 3] = opO[s].p[same];
+			c.L.buf[9 * s + 4] = opO[s].p[mixB];
+			c.L.buf[9 * s + 5] = opO[s].p[mixC];
+			c.L.buf[9 * s + 6] = opV[s].p[same];
+			c.L.buf[9 * s + 7] = opV[s].p[mixB];
+			c.L.buf[9 * s + 8] = opV[s].p[mixC];
+			c.L.inv_scale[s] = 1.0 / (pl->scale[2 * s] * pl->scale[2 * s + 1]);
+		}
+		pt_epi &e = c.L.epi;
+		e.d1 = sp == 0 ? dA : dB;
+		e.d2 = sp == 0 ? dB : dA;
+		e.u1 = sp == 0 ? uA : uB;
+		e.u2 = sp == 0 ? uB : uA;
+		e.f1 = sp == 0 ? fA : fB;
+		e.f2 = sp == 0 ? fB : fA;
+		e.GA = gv.p[same];
+		e.GB = gv.p[mixB];
+		e.ZA = zt.p[same];
+		e.ZB = zt.p[mixB];
+		c.factor = fac;
+		bool empty;
+		if (cs.kind == PT_AAA) {
+			empty = o1 < 3 || v1 < 3;
+			if (!empty)
+				PFAIL(case_finish(c, PT_AAA, nset, o1, v1, o1, v1));
+		} else {
+			empty = o1 < 2 || v1 < 2 || o2 < 1 || v2 < 1;
+			if (!empty)
+				PFAIL(case_finish(c, PT_AAB, nset, o1, v1, o2, v2));
+		}
+		pl->cases.push_back(c);
+	}
+	/* list uploads and slot clears ran on the null stream; the plan's
+	 * stream does not wait for it */
+	PFAIL(hipDeviceSynchronize() == hipSuccess ||
+	    (set_err("hipDeviceSynchronize failed"), false));
+#undef PFAIL
+	return pl;
+}
+
+extern "C" void
+libpt_amd_plan_destroy(libpt_amd_plan *pl)
+{
+	plan_free(pl);
+}
+
+extern "C" int
+libpt_amd_plan_ncases(const libpt_amd_plan *pl)
+{
+	return pl ? (int)pl->cases.size() : 0;
+}
+
+extern "C" int64_t
+libpt_amd_plan_nijk(const libpt_amd_plan *pl, int sc)
+{
+	return pl && sc >= 0 && sc < (int)pl->cases.size() ? pl->cases[sc].n_ijk : 0;
+}
+
+extern "C" int64_t
+libpt_amd_plan_items(const libpt_amd_plan *pl, int sc)
+{
+	if (!pl || sc < 0 || sc >= (int)pl->cases.size())
+		return 0;
+	return pl->cases[sc].n_ijk * pl->cases[sc].n_tiles;
+}
+
+extern "C" double
+libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *pl, int sc)
+{
+	return pl && sc >= 0 && sc < (int)pl->cases.size() ?
+	    pl->cases[sc].flop_per_ijk : 0.0;
+}
+
+extern "C" double
+libpt_amd_plan_factor(const libpt_amd_plan *pl, int sc)
+{
+	return pl && sc >= 0 && sc < (int)pl->cases.size() ?
+	    pl->cases[sc].factor : 0.0;
+}
+
+extern "C" int
+libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
+{
+	if (!pl || sc < 0 || sc >= (int)pl->cases.size()) {
+		set_err("bad plan or spin case");
+		return -1;
+	}
+	Case &c = pl->cases[sc];
+	if (c.n_ijk == 0)
+		return 0;
+	if (b < 0)
+		b = 0;
+	if (e > c.n_ijk)
+		e = c.n_ijk;
+	if (e <= b)
+		return 0;
+	const int64_t item_b = b * c.n_tiles, item_e = e * c.n_tiles;
+	const int64_t chunk = (int64_t)1 << 22;
+	for (int64_t it = item_b; it < item_e; it += chunk) {
+		pt_launch L = c.L;
+		L.item0 = it;
+		L.n_chunk = item_e - it < chunk ? item_e - it : chunk;
+		int rc = pt_launch_tiles(pl->form, &L, pl->stream);
+		if (rc != 0) {
+			set_err("tile kernel launch failed: %s",
+			    hipGetErrorString((hipError_t)rc));
+			return -1;
+		}
+	}
+	return 0;
+}
+
+extern "C" int
+libpt_amd_plan_sync(libpt_amd_plan *pl)
+{
+	if (!pl)
+		return -1;
+	HCHK(hipStreamSynchronize(pl->stream), -1);
+	return 0;
+}
+
+extern "C" double
+libpt_amd_plan_energy(libpt_amd_plan *pl)
+{
+	if (!pl)
+		return NAN;
+	double e = 0.0;
+	for (Case &c : pl->cases) {
+		if (c.n_ijk == 0)
+			continue;
+		if (pt_launch_reduce(c.d_partial, c.n_ijk * c.n_tiles, pl->d_out,
+		    pl->stream) != 0) {
+			set_err("reduce launch failed");
+			return NAN;
+		}
+		double x = 0;
+		HCHK(hipMemcpyAsync(&x, pl->d_out, sizeof(double),
+		    hipMemcpyDeviceToHost, pl->stream), NAN);
+		HCHK(hipStreamSynchronize(pl->stream), NAN);
+		e += c.factor * x;
+	}
+	return e;
+}
+
+extern "C" int
+libpt_amd_plan_ijk_energies(libpt_amd_plan *pl, int sc, double *out)
+{
+	if (!pl || sc < 0 || sc >= (int)pl->cases.size())
+		return -1;
+	Case &c = pl->cases[sc];
+	if (c.n_ijk == 0)
+		return 0;
+	double *d;
+	if (!dev_alloc((void **)&d, sizeof(double) * (size_t)c.n_ijk))
+		return -1;
+	int rc = pt_launch_segsum(c.d_partial, c.n_ijk, c.n_tiles, d, pl->stream);
+	if (rc == 0 && hipMemcpyAsync(out, d, sizeof(double) * (size_t)c.n_ijk,
+	    hipMemcpyDeviceToHost, pl->stream) == hipSuccess &&
+	    hipStreamSynchronize(pl->stream) == hipSuccess)
+		rc = 0;
+	else
+		rc = -1;
+	(void)hipFree(d);
+	return rc;
+}
+
+extern "C" int
+libpt_amd_plan_ijk_list(const libpt_amd_plan *pl, int sc, int32_t *out)
+{
+	if (!pl || sc < 0 || sc >= (int)pl->cases.size())
+		return -1;
+	const Case &c = pl->cases[sc];
+	if (c.n_ijk == 0)
+		return 0;
+	HCHK(hipMemcpy(out, c.d_ijk, 3 * sizeof(int32_t) * (size_t)c.n_ijk,
+	    hipMemcpyDeviceToHost), -1);
+	return 0;
+}
+
+extern "C" int
+libpt_amd_plan_timer_begin(libpt_amd_plan *pl)
+{
+	if (!pl)
+		return -1;
+	HCHK(hipEventRecord(pl->ev0, pl->stream), -1);
+	return 0;
+}
+
+extern "C" double
+libpt_amd_plan_timer_end_ms(libpt_amd_plan *pl)
+{
+	if (!pl)
+		return NAN;
+	float ms = 0;
+	HCHK(hipEventRecord(pl->ev1, pl->stream), NAN);
+	HCHK(hipEventSynchronize(pl->ev1), NAN);
+	HCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1), NAN);
+	return (double)ms;
+}
+
+extern "C" void *
+libpt_amd_dev_alloc(size_t bytes)
+{
+	if (!dev_init())
+		return nullptr;
+	void *p;
+	return dev_alloc(&p, bytes) ? p : nullptr;
+}
+
+extern "C" void
+libpt_amd_dev_free(void *p)
+{
+	if (p)
+		(void)hipFree(p);
+}
+
+extern "C" int
+libpt_amd_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+	if (!dev_init())
+		return -1;
+	HCHK(hipMemcpy(dst, src, bytes, (hipMemcpyKind)kind), -1);
+	return 0;
+}
+
+extern "C" int64_t
+libpt_amd_synth_size(const void *spec, int tensor)
+{
+	return pt_synth_size((const pt_synth_spec *)spec, tensor);
+}
+
+extern "C" int
+libpt_amd_synth(const void *spec_, int tensor, void *dst, int on_device,
+    int fp32)
+{
+	const pt_synth_spec *spec = (const pt_synth_spec *)spec_;
+	int64_t n = pt_synth_size(spec, tensor);
+	if (on_device) {
+		if (!dev_init())
+			return -1;
+		if (pt_launch_synth(spec, tensor, dst, n, fp32, nullptr) != 0) {
+			set_err("synth launch failed");
+			return -1;
+		}
+		HCHK(hipDeviceSynchronize(), -1);
+		return 0;
+	}
+	for (int64_t t = 0; t < n; ++t) {
+		double x = pt_synth_elem(spec, tensor, t);
+		if (fp32)
+			((float *)dst)[t] = (float)x;
+		else
+			((double *)dst)[t] = x;
+	}
+	return 0;
+}
+
+extern "C" const char *
+libpt_amd_last_error(void)
+{
+	return g_err;
+}
+
+namespace {
+
+/* Host arrays -> HBM -> plan -> energy.  Returns NaN on failure. */
+double
+host_call(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
+    const void *const *arrays, libpt_amd_stats *st)
+{
+	double t0 = now_s();
+	if (!dev_init())
+		return NAN;
+	const bool restricted = call == LIBPT_AMD_RPT || call == LIBPT_AMD_RFT;
+	if (restricted) {
+		ob = oa;
+		vb = va;
+	}
+	int64_t n[10];
+	const int narr = api_sizes(call, (int64_t)oa, (int64_t)va, (int64_t)ob,
+	    (int64_t)vb, n);
+	const size_t el = form == PT_FORM_FP32 ? 4 : 8;
+	void *d[10] = {nullptr};
+	double e = NAN;
+	libpt_amd_plan *pl = nullptr;
+	double t1 = 0, t2 = 0;
+	bool ok = true;
+	for (int q = 0; q < narr && ok; ++q) {
+		ok = dev_alloc(&d[q], el * (size_t)n[q]);
+		if (ok && n[q] > 0) {
+			hipError_t he = hipMemcpy(d[q], arrays[q], el * (size_t)n[q],
+			    hipMemcpyHostToDevice);
+			if (he != hipSuccess) {
+				set_err("host to device copy failed: %s",
+				    hipGetErrorString(he));
+				ok = false;
+			}
+		}
+	}
+	t1 = now_s();
+	if (ok)
+		pl = libpt_amd_plan_create(call, form, oa, va, ob, vb, d);
+	if (pl) {
+		t2 = now_s();
+		for (int sc = 0; sc < (int)pl->cases.size() && ok; ++sc)
+			ok = libpt_amd_plan_run(pl, sc, 0, pl->cases[sc].n_ijk) == 0;
+		if (ok)
+			e = libpt_amd_plan_energy(pl);
+	}
+	double t3 = now_s();
+	if (st) {
+		memset(st, 0, sizeof(*st));
+		st->h2d_s = t1 - t0;
+		if (pl) {
+			st->prep_s = pl->prep_s;
+			st->kernel_s = t3 - t2;
+			for (Case &c : pl->cases)
+				st->flop += c.flop_per_ijk * (double)c.n_ijk;
+			for (int s = 0; s < 4; ++s)
+				st->scale[s] = pl->scale[s];
+		}
+	}
+	plan_free(pl);
+	for (int q = 0; q < narr; ++q)
+		(void)hipFree(d[q]);
+	if (st)
+		st->wall_s = now_s() - t0;
+	return e;
+}
+
+/* pt.h semantics: a failure prints and exits with status 1, as the
+ * reference's err(1, ...) (src/pt.c:208-209, 223-224). */
+double
+dropin(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
+    const void *const *arrays)
+{
+	double e = host_call(call, form, oa, va, ob, vb, arrays, nullptr);
+	if (isnan(e) && g_err[0]) {
+		fprintf(stderr, "libpt: %s\n", g_err);
+		exit(1);
+	}
+	return e;
+}
+
+} /* namespace */
+
+extern "C" double
+libpt_amd_energy(int call, int form, size_t oa, size_t va, size_t ob,
+    size_t vb, const void *const *arrays, struct libpt_amd_stats *stats)
+{
+	g_err[0] = 0;
+	return host_call(call, form, oa, va, ob, vb, arrays, stats);
+}
+
+/* ------------------------------------------------------ pt.h drop-in */
+
+extern "C" void
+libpt_print_banner(void)
+{
+	printf("libpt-compatible perturbative triples for AMD Instinct MI355X\n");
+	printf("CCSD(T) and EOM-CCSD(fT) on CDNA4 matrix cores\n");
+}
+
+extern "C" void
+libpt_set_malloc(void *(*fn)(size_t))
+{
+	g_malloc = fn ? fn : malloc;
+}
+
+extern "C" void
+libpt_set_free(void (*fn)(void *))
+{
+	g_free = fn ? fn : free;
+}
+
+extern "C" double
+libpt_rpt(size_t oa, size_t va, const double *d_ov, const double *f_ov,
+    const double *t1, const double *t2, const double *i_oovo,
+    const double *i_oovv, const double *i_ovvv)
+{
+	const void *a[7] = {d_ov, f_ov, t1, t2, i_oovo, i_oovv, i_ovvv};
+	return dropin(LIBPT_AMD_RPT, PT_FORM_FP64, oa, va, oa, va, a);
+}
+
+extern "C" double
+libpt_rpt_mp(size_t oa, size_t va, const float *d_ov, const float *f_ov,
+    const float *t1, const float *t2, const float *i_oovo,
+    const float *i_oovv, const float *i_ovvv)
+{
+	const void *a[7] = {d_ov, f_ov, t1, t2, i_oovo, i_oovv, i_ovvv};
+	return dropin(LIBPT_AMD_RPT, PT_FORM_FP32, oa, va, oa, va, a);
+}
+
+extern "C" double
+libpt_upt(size_t oa, size_t va, size_t ob, size_t vb, const double *d_ov,
+    const double *f_ov, const double *t1, const double *t2,
+    const double *i_oovo, const double *i_oovv, const double *i_ovvv)
+{
+	const void *a[7] = {d_ov, f_ov, t1, t2, i_oovo, i_oovv, i_ovvv};
+	return dropin(LIBPT_AMD_UPT, PT_FORM_FP64, oa, va, ob, vb, a);
+}
+
+extern "C" double
+libpt_upt_mp(size_t oa, size_t va, size_t ob, size_t vb, const float *d_ov,
+    const float *f_ov, const float *t1, const float *t2, const float *i_oovo,
+    const float *i_oovv, const float *i_ovvv)
+{
+	const void *a[7] = {d_ov, f_ov, t1, t2, i_oovo, i_oovv, i_ovvv};
+	return dropin(LIBPT_AMD_UPT, PT_FORM_FP32, oa, va, ob, vb, a);
+}
+
+extern "C" double
+libpt_rft(size_t oa, size_t va, const double *d_ov, const double *f2_ov,
+    const double *l1, const double *t2, const double *l2, const double *i_oovv,
+    const double *i2_t2f2_oovo, const double *i3_ovvv, const double *i6_oovo,
+    const double *i7_ovvv)
+{
+	const void *a[10] = {d_ov, f2_ov, l1, t2, l2, i_oovv, i2_t2f2_oovo,
+	    i3_ovvv, i6_oovo, i7_ovvv};
+	return dropin(LIBPT_AMD_RFT, PT_FORM_FP64, oa, va, oa, va, a);
+}
+
+extern "C" double
+libpt_rft_mp(size_t oa, size_t va, const float *d_ov, const float *f2_ov,
+    const float *l1, const float *t2, const float *l2, const float *i_oovv,
+    const float *i2_t2f2_oovo, const float *i3_ovvv, const float *i6_oovo,
+    const float *i7_ovvv)
+{
+	const void *a[10] = {d_ov, f2_ov, l1, t2, l2, i_oovv, i2_t2f2_oovo,
+	    i3_ovvv, i6_oovo, i7_ovvv};
+	return dropin(LIBPT_AMD_RFT, PT_FORM_FP32, oa, va, oa, va, a);
+}
+
+extern "C" double
+libpt_uft(size_t oa, size_t va, size_t ob, size_t vb, const double *d_ov,
+    const double *f2_ov, const double *l1, const double *t2, const double *l2,
+    const double *i_oovv, const double *i2_t2f2_oovo, const double *i3_ovvv,
+    const double *i6_oovo, const double *i7_ovvv)
+{
+	const void *a[10] = {d_ov, f2_ov, l1, t2, l2, i_oovv, i2_t2f2_oovo,
+	    i3_ovvv, i6_oovo, i7_ovvv};
+	return dropin(LIBPT_AMD_UFT, PT_FORM_FP64, oa, va, ob, vb, a);
+}
+
+extern "C" double
+libpt_uft_mp(size_t oa, size_t va, size_t ob, size_t vb, const float *d_ov,
+    const float *f2_ov, const float *l1, const float *t2, const float *l2,
+    const float *i_oovv, const float *i2_t2f2_oovo, const float *i3_ovvv,
+    const float *i6_oovo, const float *i7_ovvv)
+{
+	const void *a[10] = {d_ov, f2_ov, l1, t2, l2, i_oovv, i2_t2f2_oovo,
+	    i3_ovvv, i6_oovo, i7_ovvv};
+	return dropin(LIBPT_AMD_UFT, PT_FORM_FP32, oa, va, ob, vb, a);
+}

@@ -1,0 +1,18 @@
+/* Term tables of the spin cases (pt_terms.cpp). */
+#ifndef LIBPT_AMD_PT_TERMS_H
+#define LIBPT_AMD_PT_TERMS_H
+
+#include <vector>
+
+#include "pt_engine.h"
+
+/* same-spin case of sizes (o, v): 18 terms for W set `set` */
+void pt_terms_same(std::vector<pt_term> &out, int set, int64_t o, int64_t v);
+/* mixed-spin case, pair spin (oa, va), odd spin (ob, vb): 16 terms */
+void pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa,
+    int64_t va, int64_t ob, int64_t vb);
+/* stable partition into phase 1 (roles a, b) then phase 2 (role c);
+ * returns the number of phase-1 terms */
+int pt_terms_order(std::vector<pt_term> &terms);
+
+#endif
