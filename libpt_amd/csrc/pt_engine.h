@@ -32,11 +32,20 @@ enum pt_role { PT_ROLE_A = 0, PT_ROLE_B = 1, PT_ROLE_C = 2 };
 #define PT_MAX_TERMS 40
 #define PT_MAX_BUFS 20
 
-/* workgroup tile: 32 a x 32 b x 16 c, four waves of 16 x 16 x 16 */
+/* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
+ * TCW is 32 only where the accumulators of one wave stay within the AGPR
+ * file: FP32 (T) (32 blocks x 4 VGPRs); FP64 blocks take 8 VGPRs, (fT)
+ * holds two W sets, and the split forms need registers for 32-deep
+ * operand prefetch, so those use 16. */
 #define PT_TA 32
 #define PT_TB 32
-#define PT_TC 16
 #define PT_THREADS 256
+
+static inline int
+pt_tcw(int form, int nset)
+{
+	return (form == 1 && nset == 1) ? 32 : 16;
+}
 
 struct pt_term {
 	int64_t s_off[3];  /* S base offset = s_off . (i, j, k) */
@@ -66,9 +75,9 @@ struct pt_launch {
 	int32_t kind, nset;
 	int64_t oa, va, ob, vb;      /* aaa: ob = oa, vb = va */
 	int32_t na, nc;              /* extents of a (= b) and c */
-	int32_t nA, nC;              /* 32-blocks over a/b, 16-blocks over c */
+	int32_t tcw, pad0_;          /* c tile width (PT_TCW_T or PT_TCW_FT) */
 	const int32_t *ijk;          /* [n_ijk][3] */
-	const int32_t *tiles;        /* [n_tiles][2]: (A block, B block), A >= B */
+	const int32_t *tiles;        /* [n_tiles][3]: blocks (A, B, C), A >= B */
 	int32_t n_tiles, pad_;
 	int64_t item0, n_chunk;      /* items [item0, item0 + n_chunk) */
 	double *partial;             /* energy of each item */

@@ -215,9 +215,12 @@ dev_alloc(void **p, size_t bytes)
 	return true;
 }
 
-/* work list of one spin case: ijk items and (A >= B) tiles */
+/* work list of one spin case: ijk items x (A, B, C) tiles.  A >= B; for
+ * the same-spin case (a > b > c) only c tiles that reach below the top of
+ * the B block. */
 bool
-case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na)
+case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na, int64_t nc,
+    int tcw)
 {
 	int64_t nijk = same ? o1 * (o1 - 1) * (o1 - 2) / 6 : o1 * (o1 - 1) / 2 * o2;
 	HostBuf<int32_t> ijk;
@@ -233,26 +236,32 @@ case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na)
 				++it;
 			}
 	int nA = (int)((na + PT_TA - 1) / PT_TA);
+	int nC = (int)((nc + tcw - 1) / tcw);
 	HostBuf<int32_t> tiles;
-	if (!tiles.alloc((size_t)nA * (nA + 1)))
+	if (!tiles.alloc(3 * (size_t)nA * (nA + 1) / 2 * nC))
 		return false;
 	int nt = 0;
 	for (int A = 0; A < nA; ++A)
-		for (int B = 0; B <= A; ++B) {
-			tiles.p[2 * nt] = A;
-			tiles.p[2 * nt + 1] = B;
-			++nt;
-		}
+		for (int B = 0; B <= A; ++B)
+			for (int Cb = 0; Cb < nC; ++Cb) {
+				/* same spin needs some b > c: c_min < b_max */
+				if (same && Cb * tcw >= B * PT_TB + PT_TB - 1)
+					continue;
+				tiles.p[3 * nt] = A;
+				tiles.p[3 * nt + 1] = B;
+				tiles.p[3 * nt + 2] = Cb;
+				++nt;
+			}
 	c.n_ijk = nijk;
 	c.n_tiles = nt;
 	if (!dev_alloc((void **)&c.d_ijk, 3 * sizeof(int32_t) * (size_t)nijk) ||
-	    !dev_alloc((void **)&c.d_tiles, 2 * sizeof(int32_t) * (size_t)nt) ||
+	    !dev_alloc((void **)&c.d_tiles, 3 * sizeof(int32_t) * (size_t)nt) ||
 	    !dev_alloc((void **)&c.d_partial,
 	    sizeof(double) * (size_t)(nijk * nt)))
 		return false;
 	HCHK(hipMemcpy(c.d_ijk, ijk.p, 3 * sizeof(int32_t) * (size_t)nijk,
 	    hipMemcpyHostToDevice), false);
-	HCHK(hipMemcpy(c.d_tiles, tiles.p, 2 * sizeof(int32_t) * (size_t)nt,
+	HCHK(hipMemcpy(c.d_tiles, tiles.p, 3 * sizeof(int32_t) * (size_t)nt,
 	    hipMemcpyHostToDevice), false);
 	HCHK(hipMemset(c.d_partial, 0, sizeof(double) * (size_t)(nijk * nt)),
 	    false);
@@ -260,8 +269,8 @@ case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na)
 }
 
 bool
-case_finish(Case &c, int kind, int nset, int64_t oa, int64_t va, int64_t ob,
-    int64_t vb)
+case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
+    int64_t ob, int64_t vb)
 {
 	pt_launch &L = c.L;
 	L.kind = kind;
@@ -272,8 +281,7 @@ case_finish(Case &c, int kind, int nset, int64_t oa, int64_t va, int64_t ob,
 	L.vb = vb;
 	L.na = (int32_t)va;
 	L.nc = (int32_t)vb;
-	L.nA = (int32_t)((va + PT_TA - 1) / PT_TA);
-	L.nC = (int32_t)((vb + PT_TC - 1) / PT_TC);
+	L.tcw = pt_tcw(form, nset);
 	for (int s = 0; s < nset; ++s) {
 		if (kind == PT_AAA)
 			pt_terms_same(c.terms, s, oa, va);
@@ -287,7 +295,7 @@ case_finish(Case &c, int kind, int nset, int64_t oa, int64_t va, int64_t ob,
 	HCHK(hipMemcpy(c.d_terms, c.terms.data(),
 	    sizeof(pt_term) * c.terms.size(), hipMemcpyHostToDevice), false);
 	L.terms = c.d_terms;
-	if (!case_lists(c, kind == PT_AAA, oa, ob, va))
+	if (!case_lists(c, kind == PT_AAA, oa, ob, va, vb, L.tcw))
 		return false;
 	L.ijk = c.d_ijk;
 	L.tiles = c.d_tiles;
@@ -522,11 +530,11 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 		if (cs.kind == PT_AAA) {
 			empty = o1 < 3 || v1 < 3;
 			if (!empty)
-				PFAIL(case_finish(c, PT_AAA, nset, o1, v1, o1, v1));
+				PFAIL(case_finish(c, form, PT_AAA, nset, o1, v1, o1, v1));
 		} else {
 			empty = o1 < 2 || v1 < 2 || o2 < 1 || v2 < 1;
 			if (!empty)
-				PFAIL(case_finish(c, PT_AAB, nset, o1, v1, o2, v2));
+				PFAIL(case_finish(c, form, PT_AAB, nset, o1, v1, o2, v2));
 		}
 		pl->cases.push_back(c);
 	}

@@ -1,8 +1,8 @@
 /*
  * CDNA4 (gfx950) kernels of the perturbative-triples engine.
  *
- * k_tiles: one workgroup = one ijk item x one (A, B) tile of 32 a x 32 b,
- * looping over 16-wide c blocks.  Four waves each own a 16 x 16 x 16
+ * k_tiles: one workgroup = one ijk item x one (A, B, C) tile of 32 a x 32 b
+ * x TCW c (TCW = 32 for (T), 16 for (fT)).  Four waves each own 16 x 16 x TCW
  * (a, b, c) block of the connected triples W in MFMA accumulators and never
  * write W to memory:
  *
@@ -51,8 +51,8 @@ template <> struct Form<PT_FORM_FP64> {
 	typedef double AE;  /* accumulator element */
 	typedef v4d ACC;
 	typedef double ER;  /* epilogue real */
-	static constexpr int NP = 1, KC = 8, KCP = 9, KS = 4;
-	static constexpr bool ROW_F64 = true, QC_HALF = false, SPLIT = false;
+	static constexpr int NP = 1, KC = 8, KS = 4;
+	static constexpr bool ROW_F64 = true, SPLIT = false;
 };
 
 template <> struct Form<PT_FORM_FP32> {
@@ -61,8 +61,8 @@ template <> struct Form<PT_FORM_FP32> {
 	typedef float AE;
 	typedef v4f ACC;
 	typedef float ER;
-	static constexpr int NP = 1, KC = 16, KCP = 17, KS = 4;
-	static constexpr bool ROW_F64 = false, QC_HALF = false, SPLIT = false;
+	static constexpr int NP = 1, KC = 16, KS = 4;
+	static constexpr bool ROW_F64 = false, SPLIT = false;
 };
 
 template <> struct Form<PT_FORM_BF16X3> {
@@ -71,25 +71,36 @@ template <> struct Form<PT_FORM_BF16X3> {
 	typedef float AE;
 	typedef v4f ACC;
 	typedef double ER;
-	static constexpr int NP = 2, KC = 32, KCP = 40, KS = 32;
-	static constexpr bool ROW_F64 = false, QC_HALF = true, SPLIT = true;
+	static constexpr int NP = 2, KC = 32, KS = 32;
+	static constexpr bool ROW_F64 = false, SPLIT = true;
 };
 
 template <> struct Form<PT_FORM_F16X3> : Form<PT_FORM_BF16X3> {};
 
+/*
+ * LDS: S rows [32][KCP] and Q rows [32 u][QP w][KCP] per plane.  Q rows are
+ * skewed (QP = 33) so 16 consecutive u at one w fall in different banks;
+ * FP64/FP32 rows are padded by one element, split rows (64 bytes of
+ * halves) have their 16-byte chunks XOR-swizzled instead.
+ */
 template <int F> struct Lds {
 	typedef Form<F> FM;
-	static constexpr int QWC = FM::QC_HALF ? 16 : 32;
-	static constexpr int S_ELEMS = FM::NP * 32 * FM::KCP;
-	static constexpr int Q_ELEMS = FM::NP * 32 * QWC * FM::KCP;
+	static constexpr int KCP = FM::SPLIT ? FM::KC : FM::KC + 1;
+	static constexpr int QP = 33;
+	static constexpr int S_PLANE = 32 * KCP;
+	static constexpr int Q_PLANE = 32 * QP * KCP;
 	static constexpr int STAGE_BYTES =
-	    (S_ELEMS + Q_ELEMS) * (int)sizeof(typename FM::L);
-	/* transpose scratch (4 waves x 4 x 16 x 17) and epilogue scratch
-	 * (4 waves x 2 sets x 16 x 64) share the staging area */
-	static constexpr int X_BYTES =
-	    4 * 2 * 16 * 64 * (int)sizeof(typename FM::AE);
+	    FM::NP * (S_PLANE + Q_PLANE) * (int)sizeof(typename FM::L);
+	static constexpr int X_BYTES = 4 * 32 * 64 * (int)sizeof(typename FM::AE);
 	static constexpr int BYTES =
 	    STAGE_BYTES > X_BYTES ? STAGE_BYTES : X_BYTES;
+	static __device__ __forceinline__ int off(int row, int k)
+	{
+		if constexpr (FM::SPLIT)
+			return row * 32 + (((k >> 3) ^ ((row >> 2) & 3)) << 3) + (k & 7);
+		else
+			return row * KCP + k;
+	}
 };
 
 /* ---------------------------------------------------------------- MFMA */
@@ -120,8 +131,6 @@ mma8(v8u a, v8u b, v4f c)
 		    0, 0, 0);
 }
 
-/* ------------------------------------------------------------- staging */
-
 template <class G>
 __device__ __forceinline__ G
 neg_if(G v, int neg)
@@ -131,27 +140,6 @@ neg_if(G v, int neg)
 	else
 		return neg ? -v : v;
 }
-
-template <int F>
-__device__ __forceinline__ void
-lds_put(typename Form<F>::L *base, int plane_elems, int idx,
-    typename Form<F>::G v)
-{
-	if constexpr (Form<F>::SPLIT) {
-		base[idx] = (uint16_t)(v & 0xffffu);
-		base[plane_elems + idx] = (uint16_t)(v >> 16);
-	} else {
-		base[idx] = v;
-	}
-}
-
-/* Role geometry of one term inside the workgroup tile. */
-struct RoleGeom {
-	int s0, sn, sext;   /* single index: origin, rows, extent */
-	int u0, uext;       /* pair index u */
-	int w0, wext, qw;   /* pair index w, LDS width */
-	int urole, wrole;
-};
 
 /* ---------------------------------------------------------- epilogue */
 
@@ -206,82 +194,250 @@ kahan(double &s, double &cmp, double x)
 	s = t;
 }
 
-/* MFMA over one staged kappa chunk into a fixed accumulator set */
-template <int F>
+/* ------------------------------------------------- one staging step */
+
+/* Operand geometry of one term inside the workgroup tile (uniform). */
+struct Step {
+	const void *S, *Q;
+	int64_t s_st, q_k, q_u, q_w;
+	int role, set, K, neg, packed, ufast;
+	int s0, sn, sext, u0, uext, w0, wext, qwe, qsh;
+};
+
+__device__ __forceinline__ Step
+make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
+    int B0, int C0, int tcw)
+{
+	Step s;
+	s.S = (const char *)p.buf[T.s_buf];
+	s.Q = (const char *)p.buf[T.q_buf];
+	s.role = T.role;
+	s.set = T.set;
+	s.K = T.K;
+	s.neg = T.sign < 0;
+	s.packed = T.packed;
+	s.ufast = T.ufast;
+	s.s_st = T.s_st;
+	s.q_k = T.q_k;
+	const int na = p.na, nc = p.nc;
+	int ur, wr;
+	if (T.role == PT_ROLE_A) {
+		s.s0 = A0; s.sn = 32; s.sext = na;
+		s.u0 = B0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
+		ur = PT_ROLE_B; wr = PT_ROLE_C;
+	} else if (T.role == PT_ROLE_B) {
+		s.s0 = B0; s.sn = 32; s.sext = na;
+		s.u0 = A0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
+		ur = PT_ROLE_A; wr = PT_ROLE_C;
+	} else {
+		s.s0 = C0; s.sn = tcw; s.sext = nc;
+		s.u0 = A0; s.uext = na; s.w0 = B0; s.wext = na; s.qwe = 32;
+		ur = PT_ROLE_A; wr = PT_ROLE_B;
+	}
+	s.qsh = s.qwe == 32 ? 5 : 4;
+	s.q_u = T.q_st[ur];
+	s.q_w = T.q_st[wr];
+	s.S = (const char *)s.S; /* offsets applied by the caller's type */
+	return s;
+}
+
+/* pairs (u, w) of this thread for a step: pair index tid + 256 m */
 __device__ __forceinline__ void
-mma_chunk(typename Form<F>::ACC (&ac)[16], int role,
+pair_uw(const Step &s, int pidx, int &u, int &w)
+{
+	if (s.ufast) {
+		u = pidx & 31;
+		w = pidx >> 5;
+	} else {
+		w = pidx & (s.qwe - 1);
+		u = pidx >> s.qsh;
+	}
+}
+
+template <int F, int NPR, int SEPT>
+__device__ __forceinline__ void
+load_step(const Step &s, int64_t soff, int64_t qoff, int kc, int tid,
+    typename Form<F>::G (&qr)[NPR][Form<F>::KC],
+    typename Form<F>::G (&sr)[SEPT])
+{
+	typedef typename Form<F>::G G;
+	constexpr int KC = Form<F>::KC;
+	const G *S = (const G *)s.S + soff;
+	const G *Q = (const G *)s.Q + qoff;
+	/* S: thread owns SEPT consecutive kappa of one row */
+	{
+		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+		const int sg = s.s0 + row;
+		const bool rv = row < s.sn && sg < s.sext;
+		const G *sp = S + (int64_t)(rv ? sg : 0) * s.s_st;
+#pragma unroll
+		for (int e = 0; e < SEPT; ++e) {
+			const int kg = kc + k0 + e;
+			sr[e] = (rv && kg < s.K) ? neg_if(sp[kg], s.neg) : G(0);
+		}
+	}
+	const int npr = (32 * s.qwe) >> 8;
+#pragma unroll
+	for (int m = 0; m < NPR; ++m) {
+		if (m >= npr)
+			break;
+		int u, w;
+		pair_uw(s, tid + 256 * m, u, w);
+		const int ug = s.u0 + u, wg = s.w0 + w;
+		const bool v = ug < s.uext && wg < s.wext && (!s.packed || ug > wg);
+		int64_t base = 0;
+		if (v)
+			base = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
+			    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
+		const G *qp = Q + base + (int64_t)kc * s.q_k;
+		const int kmax = v ? s.K - kc : 0;
+#pragma unroll
+		for (int kk = 0; kk < KC; ++kk)
+			qr[m][kk] = kk < kmax ? qp[(int64_t)kk * s.q_k] : G(0);
+	}
+}
+
+template <int F, int NPR, int SEPT>
+__device__ __forceinline__ void
+store_step(const Step &s, int tid, typename Form<F>::L *sS,
+    typename Form<F>::L *sQ, const typename Form<F>::G (&qr)[NPR][Form<F>::KC],
+    const typename Form<F>::G (&sr)[SEPT])
+{
+	typedef Form<F> FM;
+	typedef Lds<F> LD;
+	constexpr int KC = FM::KC;
+	{
+		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+#pragma unroll
+		for (int e = 0; e < SEPT; ++e) {
+			const int o = LD::off(row, k0 + e);
+			if constexpr (FM::SPLIT) {
+				sS[o] = (uint16_t)(sr[e] & 0xffffu);
+				sS[LD::S_PLANE + o] = (uint16_t)(sr[e] >> 16);
+			} else {
+				sS[o] = sr[e];
+			}
+		}
+	}
+	const int npr = (32 * s.qwe) >> 8;
+#pragma unroll
+	for (int m = 0; m < NPR; ++m) {
+		if (m >= npr)
+			break;
+		int u, w;
+		pair_uw(s, tid + 256 * m, u, w);
+		const int row = u * LD::QP + w;
+		if constexpr (FM::SPLIT) {
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				v8u h, l;
+#pragma unroll
+				for (int e = 0; e < 8; ++e) {
+					h[e] = (uint16_t)(qr[m][8 * q + e] & 0xffffu);
+					l[e] = (uint16_t)(qr[m][8 * q + e] >> 16);
+				}
+				const int o = LD::off(row, 8 * q);
+				*(v8u *)(sQ + o) = h;
+				*(v8u *)(sQ + LD::Q_PLANE + o) = l;
+			}
+		} else {
+#pragma unroll
+			for (int kk = 0; kk < KC; ++kk)
+				sQ[LD::off(row, kk)] = qr[m][kk];
+		}
+	}
+}
+
+/* MFMA over one staged chunk into a fixed accumulator set */
+template <int F, int NCB>
+__device__ __forceinline__ void
+mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role,
     const typename Form<F>::L *sS, const typename Form<F>::L *sQ, int wa,
     int wb, int ln, int lg)
 {
 	typedef Form<F> FM;
+	typedef Lds<F> LD;
 	typedef typename FM::L L;
-	constexpr int KC = FM::KC, KCP = FM::KCP, KS = FM::KS;
-	constexpr int QWC = Lds<F>::QWC;
-	constexpr int SPE = 32 * KCP, QPE = 32 * QWC * KCP;
-	(void)SPE; (void)QPE; (void)KS; (void)KC;
+	constexpr int KC = FM::KC, KS = FM::KS, QP = LD::QP;
+	constexpr int TCW = 16 * NCB;
 	if constexpr (!FM::SPLIT) {
 #pragma unroll
 		for (int ks = 0; ks < KC; ks += KS) {
 			const int kap = ks + lg;
 			if (role == PT_ROLE_A) {
-				L av = sS[(16 * wa + ln) * KCP + kap];
+				const L av = sS[LD::off(16 * wa + ln, kap)];
+				const int r0 = (16 * wb + ln) * QP;
 #pragma unroll
-				for (int c = 0; c < 16; ++c) {
-					L bv = sQ[((16 * wb + ln) * 16 + c) * KCP + kap];
-					ac[c] = mma(av, bv, ac[c]);
-				}
+				for (int x = 0; x < TCW; ++x)
+					ac[x] = mma(av, sQ[LD::off(r0 + x, kap)], ac[x]);
 			} else if (role == PT_ROLE_B) {
-				L bv = sS[(16 * wb + ln) * KCP + kap];
+				const L bv = sS[LD::off(16 * wb + ln, kap)];
+				const int r0 = (16 * wa + ln) * QP;
 #pragma unroll
-				for (int c = 0; c < 16; ++c) {
-					L av = sQ[((16 * wa + ln) * 16 + c) * KCP + kap];
-					ac[c] = mma(av, bv, ac[c]);
-				}
+				for (int x = 0; x < TCW; ++x)
+					ac[x] = mma(sQ[LD::off(r0 + x, kap)], bv, ac[x]);
 			} else {
-				L bv = sS[ln * KCP + kap];
+				L bv[NCB];
+#pragma unroll
+				for (int cb = 0; cb < NCB; ++cb)
+					bv[cb] = sS[LD::off(16 * cb + ln, kap)];
+				const int r0 = (16 * wa + ln) * QP + 16 * wb;
 #pragma unroll
 				for (int b = 0; b < 16; ++b) {
-					L av = sQ[((16 * wa + ln) * QWC + 16 * wb + b) * KCP + kap];
-					ac[b] = mma(av, bv, ac[b]);
+					const L av = sQ[LD::off(r0 + b, kap)];
+#pragma unroll
+					for (int cb = 0; cb < NCB; ++cb)
+						ac[16 * cb + b] = mma(av, bv[cb], ac[16 * cb + b]);
 				}
 			}
 		}
 	} else {
-		const int ko = 8 * lg;
-		const int wq = QWC == 16 ? 0 : 16 * wb;
-#define LD8(base, row) (*(const v8u *)((base) + (row) * KCP + ko))
+#define LD8(base, plane, row) \
+	(*(const v8u *)((base) + (plane) + (row) * 32 + \
+	    ((lg ^ (((row) >> 2) & 3)) << 3)))
 		if (role == PT_ROLE_A) {
 			const int rs = 16 * wa + ln;
-			v8u ah = LD8(sS, rs), al = LD8(sS + SPE, rs);
+			const v8u ah = LD8(sS, 0, rs), al = LD8(sS, LD::S_PLANE, rs);
+			const int r0 = (16 * wb + ln) * QP;
 #pragma unroll
-			for (int c = 0; c < 16; ++c) {
-				const int rq = (16 * wb + ln) * 16 + c;
-				v8u bh = LD8(sQ, rq), bl = LD8(sQ + QPE, rq);
-				ac[c] = mma8<F>(al, bh, ac[c]);
-				ac[c] = mma8<F>(ah, bl, ac[c]);
-				ac[c] = mma8<F>(ah, bh, ac[c]);
+			for (int x = 0; x < TCW; ++x) {
+				const v8u bh = LD8(sQ, 0, r0 + x);
+				const v8u bl = LD8(sQ, LD::Q_PLANE, r0 + x);
+				ac[x] = mma8<F>(al, bh, ac[x]);
+				ac[x] = mma8<F>(ah, bl, ac[x]);
+				ac[x] = mma8<F>(ah, bh, ac[x]);
 			}
 		} else if (role == PT_ROLE_B) {
 			const int rs = 16 * wb + ln;
-			v8u bh = LD8(sS, rs), bl = LD8(sS + SPE, rs);
+			const v8u bh = LD8(sS, 0, rs), bl = LD8(sS, LD::S_PLANE, rs);
+			const int r0 = (16 * wa + ln) * QP;
 #pragma unroll
-			for (int c = 0; c < 16; ++c) {
-				const int rq = (16 * wa + ln) * 16 + c;
-				v8u ah = LD8(sQ, rq), al = LD8(sQ + QPE, rq);
-				ac[c] = mma8<F>(al, bh, ac[c]);
-				ac[c] = mma8<F>(ah, bl, ac[c]);
-				ac[c] = mma8<F>(ah, bh, ac[c]);
+			for (int x = 0; x < TCW; ++x) {
+				const v8u ah = LD8(sQ, 0, r0 + x);
+				const v8u al = LD8(sQ, LD::Q_PLANE, r0 + x);
+				ac[x] = mma8<F>(al, bh, ac[x]);
+				ac[x] = mma8<F>(ah, bl, ac[x]);
+				ac[x] = mma8<F>(ah, bh, ac[x]);
 			}
 		} else {
-			const int rs = ln;
-			v8u bh = LD8(sS, rs), bl = LD8(sS + SPE, rs);
+			v8u bh[NCB], bl[NCB];
+#pragma unroll
+			for (int cb = 0; cb < NCB; ++cb) {
+				bh[cb] = LD8(sS, 0, 16 * cb + ln);
+				bl[cb] = LD8(sS, LD::S_PLANE, 16 * cb + ln);
+			}
+			const int r0 = (16 * wa + ln) * QP + 16 * wb;
 #pragma unroll
 			for (int b = 0; b < 16; ++b) {
-				const int rq = (16 * wa + ln) * QWC + wq + b;
-				v8u ah = LD8(sQ, rq), al = LD8(sQ + QPE, rq);
-				ac[b] = mma8<F>(al, bh, ac[b]);
-				ac[b] = mma8<F>(ah, bl, ac[b]);
-				ac[b] = mma8<F>(ah, bh, ac[b]);
+				const v8u ah = LD8(sQ, 0, r0 + b);
+				const v8u al = LD8(sQ, LD::Q_PLANE, r0 + b);
+#pragma unroll
+				for (int cb = 0; cb < NCB; ++cb) {
+					const int x = 16 * cb + b;
+					ac[x] = mma8<F>(al, bh[cb], ac[x]);
+					ac[x] = mma8<F>(ah, bl[cb], ac[x]);
+					ac[x] = mma8<F>(ah, bh[cb], ac[x]);
+				}
 			}
 		}
 #undef LD8
@@ -295,28 +451,30 @@ __global__ __launch_bounds__(PT_THREADS, 1) void
 k_tiles(pt_launch p)
 {
 	typedef Form<F> FM;
+	typedef Lds<F> LD;
 	typedef typename FM::G G;
 	typedef typename FM::L L;
 	typedef typename FM::AE AE;
 	typedef typename FM::ACC ACC;
 	typedef typename FM::ER ER;
-	constexpr int NP = FM::NP, KC = FM::KC, KCP = FM::KCP, KS = FM::KS;
-	constexpr int QWC = Lds<F>::QWC;
-	constexpr int SPE = 32 * KCP;          /* S plane elements */
-	constexpr int QPE = 32 * QWC * KCP;    /* Q plane elements */
+	constexpr int KC = FM::KC;
+	constexpr int NCB = (F == PT_FORM_FP32 && NSET == 1) ? 2 : 1; /* pt_tcw */
+	constexpr int TCW = 16 * NCB;
+	constexpr int NPR = 4;                     /* (u, w) pairs per thread */
+	constexpr int SEPT = 32 * KC / PT_THREADS; /* S elements per thread */
 
-	__shared__ __attribute__((aligned(16))) char smem[Lds<F>::BYTES];
+	__shared__ __attribute__((aligned(16))) char smem[LD::BYTES];
 	__shared__ double red[4];
 	L *sS = (L *)smem;
-	L *sQ = sS + NP * SPE;
+	L *sQ = sS + FM::NP * LD::S_PLANE;
 	AE *sX = (AE *)smem;
 
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int wa = wave & 1, wb = wave >> 1;
 	const int ln = lane & 15, lg = lane >> 4;
 
-	/* XCD-aware item order: workgroup n runs on XCD n % 8; give each XCD
-	 * a contiguous run of items so items sharing an ijk share its L2. */
+	/* XCD-aware item order: workgroup n runs on XCD n % 8; each XCD gets a
+	 * contiguous run of items, so neighbouring tiles of one ijk share L2 */
 	const int64_t per = (p.n_chunk + 7) / 8;
 	int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 	if (it >= p.n_chunk)
@@ -326,184 +484,138 @@ k_tiles(pt_launch p)
 	const int tile = (int)(it - ijk_id * p.n_tiles);
 	const int i = p.ijk[3 * ijk_id], j = p.ijk[3 * ijk_id + 1];
 	const int k = p.ijk[3 * ijk_id + 2];
-	const int Ab = p.tiles[2 * tile], Bb = p.tiles[2 * tile + 1];
-	const int A0 = Ab * 32, B0 = Bb * 32;
+	const int A0 = p.tiles[3 * tile] * 32, B0 = p.tiles[3 * tile + 1] * 32;
+	const int C0 = p.tiles[3 * tile + 2] * TCW;
 	const int na = p.na, nc = p.nc;
-	const int cmax = KIND == PT_AAA ? min(2 * Bb + 1, p.nC - 1) : p.nC - 1;
 	const int a_lo = A0 + 16 * wa, b_lo = B0 + 16 * wb;
+	/* does this wave's block hold any element of the triangle? */
+	const bool act = a_lo < na && b_lo < na && a_lo + 15 > b_lo &&
+	    (KIND != PT_AAA || b_lo + 15 > C0);
 
+	ACC acc[NSET][TCW];
+#pragma unroll
+	for (int s = 0; s < NSET; ++s)
+#pragma unroll
+		for (int x = 0; x < TCW; ++x)
+			acc[s][x] = ACC{0, 0, 0, 0};
+
+	/* steps = (term, kappa chunk) over all terms; phase 1 first */
+	const int nterm = p.nterm1 + p.nterm2;
+	G qr[NPR][KC], sr[SEPT];
+	int t = 0, kc = 0;
+	Step cur = make_step(p, p.terms[0], i, j, k, A0, B0, C0, TCW);
+	int64_t soff = 0, qoff = 0;
+	{
+		const pt_term &T = p.terms[0];
+		soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
+		qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+	}
+	load_step<F, NPR, SEPT>(cur, soff, qoff, 0, tid, qr, sr);
+
+	while (t < nterm) {
+		/* phase boundary: [a][b]@c -> [a][c]@b, per register r */
+		if (t == p.nterm1 && kc == 0) {
+#pragma unroll
+			for (int s = 0; s < NSET; ++s)
+#pragma unroll
+			for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				AE *X = sX + wave * (4 * 16 * 17);
+				__syncthreads();
+#pragma unroll
+				for (int c = 0; c < 16; ++c)
+					X[(lg * 16 + ln) * 17 + c] = acc[s][16 * cb + c][r];
+				__syncthreads();
+#pragma unroll
+				for (int b = 0; b < 16; ++b)
+					acc[s][16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
+			}
+		}
+		__syncthreads();
+		store_step<F, NPR, SEPT>(cur, tid, sS, sQ, qr, sr);
+		__syncthreads();
+		/* prefetch the next step while this one computes */
+		const int role = cur.role, set = cur.set;
+		int tn = t, kn = kc + KC;
+		if (kn >= cur.K) {
+			tn = t + 1;
+			kn = 0;
+		}
+		if (tn < nterm) {
+			if (tn != t) {
+				const pt_term &T = p.terms[tn];
+				cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
+				soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
+				qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+			}
+			load_step<F, NPR, SEPT>(cur, soff, qoff, kn, tid, qr, sr);
+		}
+		if (act) {
+			if (NSET == 1 || set == 0)
+				mma_step<F, NCB>(acc[0], role, sS, sQ, wa, wb, ln, lg);
+			else
+				mma_step<F, NCB>(acc[NSET - 1], role, sS, sQ, wa, wb, ln,
+				    lg);
+		}
+		t = tn;
+		kc = kn;
+	}
+
+	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
+	 * c = C0 + 16 cb + ln.  Each register column r goes through this
+	 * lane's own LDS slots so the element loop runs with a runtime index. */
 	double esum = 0.0, ecmp = 0.0;
-
-	for (int Cb = 0; Cb <= cmax; ++Cb) {
-		const int C0 = Cb * 16;
-		ACC acc[NSET][16];
+	AE *E = sX + wave * (NSET * TCW * 64) + lane;
+#pragma unroll
+	for (int r = 0; r < 4; ++r) {
+		__syncthreads();
 #pragma unroll
 		for (int s = 0; s < NSET; ++s)
 #pragma unroll
-			for (int x = 0; x < 16; ++x)
-				acc[s][x] = ACC{0, 0, 0, 0};
-		/* does this wave's 16^3 block hold any element of the triangle? */
-		const bool act = a_lo < na && b_lo < na && a_lo + 15 > b_lo &&
-		    (KIND != PT_AAA || b_lo + 15 > C0);
-
-		for (int ph = 1; ph <= 2; ++ph) {
-			const int t0 = ph == 1 ? 0 : p.nterm1;
-			const int t1 = ph == 1 ? p.nterm1 : p.nterm1 + p.nterm2;
-			if (ph == 2) {
-				/* [a][b]@c -> [a][c]@b, per register r */
-#pragma unroll
-				for (int s = 0; s < NSET; ++s)
-#pragma unroll
-				for (int r = 0; r < 4; ++r) {
-					AE *X = sX + wave * (4 * 16 * 17);
-					__syncthreads();
-#pragma unroll
-					for (int c = 0; c < 16; ++c)
-						X[(lg * 16 + ln) * 17 + c] = acc[s][c][r];
-					__syncthreads();
-#pragma unroll
-					for (int b = 0; b < 16; ++b)
-						acc[s][b][r] = X[(lg * 16 + b) * 17 + ln];
-				}
+			for (int x = 0; x < TCW; ++x)
+				E[(s * TCW + x) * 64] = acc[s][x][r];
+		if (!act)
+			continue;
+		const int arow = FM::ROW_F64 ? (lg + 4 * r) : (4 * lg + r);
+		const int a = a_lo + arow;
+		if (a >= na)
+			continue;
+		for (int x = 0; x < TCW; ++x) {
+			const int b = b_lo + (x & 15);
+			const int c = C0 + 16 * (x >> 4) + ln;
+			if (!(c < nc && a > b && (KIND != PT_AAA || b > c)))
+				continue;
+			ER z, dn;
+			const ER *d1 = (const ER *)p.epi.d1;
+			if (KIND == PT_AAA) {
+				const int64_t o = p.oa, v = p.va;
+				z = z_same<ER>((const ER *)p.epi.u1, (const ER *)p.epi.GA,
+				    o, v, i, j, k, a, b, c) +
+				    z_same<ER>((const ER *)p.epi.f1, (const ER *)p.epi.ZA,
+				    o, v, i, j, k, a, b, c);
+				dn = d1[i * v + a] + d1[j * v + b] + d1[k * v + c];
+			} else {
+				const ER *d2 = (const ER *)p.epi.d2;
+				z = z_mixed<ER>(p.epi, p.oa, p.va, p.ob, p.vb, i, j, k, a, b,
+				    c);
+				dn = d1[i * p.va + a] + d1[j * p.va + b] + d2[k * p.vb + c];
 			}
-			for (int t = t0; t < t1; ++t) {
-				const pt_term &T = p.terms[t];
-				const int role = T.role, set = T.set, K = T.K;
-				const int neg = T.sign < 0, packed = T.packed;
-				const int ufast = T.ufast;
-				const G *S = (const G *)p.buf[T.s_buf] +
-				    T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
-				const G *Q = (const G *)p.buf[T.q_buf] +
-				    T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
-				const int64_t s_st = T.s_st, q_k = T.q_k;
-				RoleGeom g;
-				if (role == PT_ROLE_A) {
-					g = RoleGeom{A0, 32, na, B0, na, C0, nc, 16,
-					    PT_ROLE_B, PT_ROLE_C};
-				} else if (role == PT_ROLE_B) {
-					g = RoleGeom{B0, 32, na, A0, na, C0, nc, 16,
-					    PT_ROLE_A, PT_ROLE_C};
-				} else {
-					g = RoleGeom{C0, 16, nc, A0, na, B0, na, QWC,
-					    PT_ROLE_A, PT_ROLE_B};
+			ER term;
+			if (NSET == 1) {
+				ER w = (ER)E[x * 64];
+				if (FM::SPLIT)
+					w *= (ER)p.inv_scale[0];
+				term = w * (w - z) / dn;
+			} else {
+				ER wl = (ER)E[x * 64], wr = (ER)E[(TCW + x) * 64];
+				if (FM::SPLIT) {
+					wl *= (ER)p.inv_scale[0];
+					wr *= (ER)p.inv_scale[1];
 				}
-				const int64_t q_u = T.q_st[g.urole], q_w = T.q_st[g.wrole];
-				const int nhalf = (role == PT_ROLE_C && FM::QC_HALF) ? 2 : 1;
-
-				for (int h = 0; h < nhalf; ++h) {
-				const int w0 = g.w0 + 16 * h;
-				for (int kc = 0; kc < K; kc += KC) {
-					__syncthreads();
-					/* S rows [s][kappa], sign folded in */
-					for (int e = tid; e < 32 * KC; e += PT_THREADS) {
-						const int s = e / KC, kk = e % KC;
-						const int sg = g.s0 + s, kg = kc + kk;
-						G v = G(0);
-						if (s < g.sn && sg < g.sext && kg < K)
-							v = S[(int64_t)sg * s_st + kg];
-						lds_put<F>(sS, SPE, s * KCP + kk, neg_if(v, neg));
-					}
-					/* Q tile [u][w][kappa] */
-					const int nq = 32 * g.qw * KC;
-					const int qsh = g.qw == 32 ? 5 : 4;
-					for (int e = tid; e < nq; e += PT_THREADS) {
-						int u, w, kk;
-						if (ufast) {
-							u = e & 31;
-							w = (e >> 5) & (g.qw - 1);
-							kk = (e >> 5) >> qsh;
-						} else {
-							w = e & (g.qw - 1);
-							u = (e >> qsh) & 31;
-							kk = (e >> qsh) >> 5;
-						}
-						const int ug = g.u0 + u, wg = w0 + w, kg = kc + kk;
-						G v = G(0);
-						if (ug < g.uext && wg < g.wext && kg < K &&
-						    (!packed || ug > wg)) {
-							int64_t off = (int64_t)kg * q_k;
-							if (packed)
-								off += (int64_t)ug * (ug - 1) / 2 + wg;
-							else
-								off += (int64_t)ug * q_u +
-								    (int64_t)wg * q_w;
-							v = Q[off];
-						}
-						lds_put<F>(sQ, QPE, (u * g.qw + w) * KCP + kk, v);
-					}
-					__syncthreads();
-					if (!act)
-						continue;
-					if (role == PT_ROLE_C && FM::QC_HALF && wb != h)
-						continue;
-					if (NSET == 1 || set == 0)
-						mma_chunk<F>(acc[0], role, sS, sQ, wa, wb, ln, lg);
-					else
-						mma_chunk<F>(acc[NSET - 1], role, sS, sQ, wa, wb,
-						    ln, lg);
-				}
-				}
+				term = (wl - z) * wr / dn;
 			}
-		}
-
-		/* epilogue: layout [a][c]@b; lane holds c = C0 + ln.  Each
-		 * register column r goes through this lane's own LDS slots so
-		 * the element loop can run with a runtime index. */
-		{
-			const int c = C0 + ln;
-			AE *E = sX + wave * (NSET * 16 * 64) + lane;
-#pragma unroll
-			for (int r = 0; r < 4; ++r) {
-				__syncthreads();
-#pragma unroll
-				for (int s = 0; s < NSET; ++s)
-#pragma unroll
-					for (int bb = 0; bb < 16; ++bb)
-						E[(s * 16 + bb) * 64] = acc[s][bb][r];
-				if (!act)
-					continue;
-				const int arow = FM::ROW_F64 ? (lg + 4 * r) : (4 * lg + r);
-				const int a = a_lo + arow;
-				if (a >= na || c >= nc)
-					continue;
-				for (int bb = 0; bb < 16; ++bb) {
-					const int b = b_lo + bb;
-					if (!(a > b && (KIND != PT_AAA || b > c)))
-						continue;
-					ER z, dn;
-					const ER *d1 = (const ER *)p.epi.d1;
-					if (KIND == PT_AAA) {
-						const int64_t o = p.oa, v = p.va;
-						z = z_same<ER>((const ER *)p.epi.u1,
-						    (const ER *)p.epi.GA, o, v, i, j, k, a, b, c) +
-						    z_same<ER>((const ER *)p.epi.f1,
-						    (const ER *)p.epi.ZA, o, v, i, j, k, a, b, c);
-						dn = d1[i * v + a] + d1[j * v + b] + d1[k * v + c];
-					} else {
-						const ER *d2 = (const ER *)p.epi.d2;
-						z = z_mixed<ER>(p.epi, p.oa, p.va, p.ob, p.vb,
-						    i, j, k, a, b, c);
-						dn = d1[i * p.va + a] + d1[j * p.va + b] +
-						    d2[k * p.vb + c];
-					}
-					ER term;
-					if (NSET == 1) {
-						ER w = (ER)E[bb * 64];
-						if (FM::SPLIT)
-							w *= (ER)p.inv_scale[0];
-						term = w * (w - z) / dn;
-					} else {
-						ER wl = (ER)E[bb * 64];
-						ER wr = (ER)E[(16 + bb) * 64];
-						if (FM::SPLIT) {
-							wl *= (ER)p.inv_scale[0];
-							wr *= (ER)p.inv_scale[1];
-						}
-						term = (wl - z) * wr / dn;
-					}
-					kahan(esum, ecmp, (double)term);
-				}
-			}
+			kahan(esum, ecmp, (double)term);
 		}
 	}
 
