@@ -129,12 +129,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--o", type=int, default=45)
     ap.add_argument("--v", type=int, default=477)
-    ap.add_argument("--nbatch", type=int, default=64)
+    ap.add_argument("--nbatch", type=int, default=128)
     ap.add_argument("--forms", default="fp64,bf16x3")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu", default="16,160",
                     help="o,v of the CPU-baseline sample; 'none' to skip")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a device copy of this many GB (for scaling "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        from libpt_amd import api
+        n = int(args.copy_calibration * 1e9) // 8
+        t0 = time.perf_counter()
+        api.device_copy_calibration(n)
+        print(json.dumps({"copy_calibration_bytes": 8 * n,
+                          "s": time.perf_counter() - t0}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is "
               "not supported (the ijk work list would be split across GPUs "

@@ -97,6 +97,10 @@ int libpt_amd_synth(const void *spec, int tensor, void *dst, int on_device,
     int fp32);
 int64_t libpt_amd_synth_size(const void *spec, int tensor);
 
+/* copy n doubles device-to-device with a plain kernel (reference point
+ * for hardware traffic counters) */
+int libpt_amd_copy_kernel(void *dst, const void *src, int64_t n);
+
 /* message of the last failed call of this API */
 const char *libpt_amd_last_error(void);
 

@@ -108,6 +108,8 @@ def lib():
         L.libpt_amd_synth.argtypes = [C.POINTER(SynthSpec), _i, _p, _i, _i]
         L.libpt_amd_synth_size.restype = _i64
         L.libpt_amd_synth_size.argtypes = [C.POINTER(SynthSpec), _i]
+        L.libpt_amd_copy_kernel.restype = _i
+        L.libpt_amd_copy_kernel.argtypes = [_p, _p, _i64]
         L.libpt_amd_last_error.restype = C.c_char_p
         L.libpt_print_banner.argtypes = []
         L.libpt_set_malloc.argtypes = [_p]
@@ -346,6 +348,23 @@ class Plan:
 
     def __exit__(self, *a):
         self.close()
+
+
+def device_copy_calibration(n_doubles: int) -> None:
+    """Copy n doubles device-to-device with the engine's own copy kernel
+    (reads 8n bytes, writes 8n bytes): the reference point for scaling
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    L = lib()
+    a = L.libpt_amd_dev_alloc(8 * n_doubles)
+    b = L.libpt_amd_dev_alloc(8 * n_doubles)
+    if not a or not b:
+        raise LibptError(last_error())
+    try:
+        if L.libpt_amd_copy_kernel(b, a, n_doubles) != 0:
+            raise LibptError(last_error())
+    finally:
+        L.libpt_amd_dev_free(a)
+        L.libpt_amd_dev_free(b)
 
 
 def flop(kind, dims):

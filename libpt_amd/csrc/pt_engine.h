@@ -30,7 +30,21 @@ enum pt_kind { PT_AAA = 0, PT_AAB = 1 };
 enum pt_role { PT_ROLE_A = 0, PT_ROLE_B = 1, PT_ROLE_C = 2 };
 
 #define PT_MAX_TERMS 40
-#define PT_MAX_BUFS 20
+#define PT_MAX_BUFS 24
+
+/* kappa block of the packed Q operands per form (one block of one (u, w)
+ * pair is 64 bytes: 8 FP64, 16 FP32, or 32 hi + 32 lo halves = 128 bytes) */
+static inline int
+pt_kc(int form)
+{
+	return form == 0 ? 8 : form == 1 ? 16 : 32;
+}
+
+static inline int
+pt_group_bytes(int form)
+{
+	return form >= 2 ? 128 : 64;
+}
 
 /* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
  * TCW is 32 only where the accumulators of one wave stay within the AGPR
@@ -47,12 +61,19 @@ pt_tcw(int form, int nset)
 	return (form == 1 && nset == 1) ? 32 : 16;
 }
 
+/*
+ * S operands are read in the API layout (kappa contiguous).  Q operands
+ * [slab][kappa][plane] are read from packed copies
+ * [slab][kappa / KC][plane][kappa % KC] (zero padded in kappa), so the KC
+ * values of one (u, w) pair form one contiguous group.
+ */
 struct pt_term {
-	int64_t s_off[3];  /* S base offset = s_off . (i, j, k) */
-	int64_t q_off[3];  /* Q base offset = q_off . (i, j, k) */
+	int64_t s_off[3];  /* S base offset (elements) = s_off . (i, j, k) */
+	int64_t q_off[3];  /* Q slab offset: in the API layout (elements) when
+	                      built, in bytes of the packed copy when launched */
 	int64_t s_st;      /* stride of the single index s (kappa stride 1) */
-	int64_t q_k;       /* stride of kappa in Q */
-	int64_t q_st[3];   /* strides of a, b, c in Q (the S role's unused) */
+	int64_t q_k;       /* plane size of Q (kappa stride in the API layout) */
+	int64_t q_st[3];   /* strides of a, b, c in the Q plane (S role unused) */
 	int32_t s_buf, q_buf;
 	int32_t role, set, sign, K;
 	int32_t packed;    /* Q pair (u > w) is a packed triangle u(u-1)/2 + w */
@@ -95,7 +116,12 @@ int pt_launch_segsum(const double *x, int64_t nseg, int64_t seglen,
     double *out, void *stream);
 int pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
     double scale, void *stream);
+/* [R][K][PL] -> [R][ceil(K/KC)][PL][KC] (mode 0 FP64, 1 FP32) or
+ * [R][ceil(K/KC)][PL][hi KC][lo KC] halves (mode 2 BF16, 3 FP16, of x*scale) */
+int pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K,
+    int64_t PL, int KC, int mode, double scale, void *stream);
 int pt_launch_absmax(const double *src, int64_t n, double *out, void *stream);
+int pt_launch_copy(const double *src, double *dst, int64_t n, void *stream);
 int pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n,
     int fp32, void *stream);
 #ifdef __cplusplus

@@ -288,6 +288,15 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 		else
 			pt_terms_mixed(c.terms, s, oa, va, ob, vb);
 	}
+	/* Q slab offsets: API elements (K q_k per slab) -> bytes of the
+	 * packed copy (ceil(K / KC) q_k groups per slab) */
+	for (pt_term &t : c.terms) {
+		const int64_t slab = (int64_t)t.K * t.q_k;
+		const int64_t pslab = ((t.K + pt_kc(form) - 1) / pt_kc(form)) * t.q_k *
+		    pt_group_bytes(form);
+		for (int x = 0; x < 3; ++x)
+			t.q_off[x] = t.q_off[x] / slab * pslab;
+	}
 	L.nterm1 = pt_terms_order(c.terms);
 	L.nterm2 = (int32_t)c.terms.size() - L.nterm1;
 	if (!dev_alloc((void **)&c.d_terms, sizeof(pt_term) * c.terms.size()))
@@ -371,58 +380,108 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	const size_t esz = fp32 ? 4 : 8;     /* epilogue element size */
 	const size_t osz = fp32 || split ? 4 : 8; /* operand element size */
 
-	/* operand arrays: (T) t2, oovo, ovvv; (fT) t2, l2, i2, i3, i6, i7 */
-	const int *opidx;
-	static const int op_t[3] = {3, 4, 6};
-	static const int op_f[6] = {3, 4, 6, 7, 8, 9};
-	const int nop = ft ? 6 : 3;
-	opidx = ft ? op_f : op_t;
-	const void *oparr[10];
-	for (int q = 0; q < 10; ++q)
-		oparr[q] = q < (ft ? 10 : 7) ? arr[q] : nullptr;
-	if (split) {
-		double t0 = now_s();
-		/* W sets: (T) one set {t2 | oovo, ovvv};
-		 * (fT) set 0 = L {l2 | i6, i7}, set 1 = R {t2 | i2, i3} */
-		const int nset = ft ? 2 : 1;
-		for (int s = 0; s < nset; ++s) {
-			int amp, i1, i2;
-			if (!ft) {
-				amp = 3; i1 = 4; i2 = 6;
-			} else if (s == 0) {
-				amp = 4; i1 = 8; i2 = 9;
-			} else {
-				amp = 3; i1 = 6; i2 = 7;
+	/* per W set, API array index of the amplitudes (S operand of ovvv
+	 * terms, Q operand of oovo terms), the oovo integrals (S operand) and
+	 * the ovvv integrals (Q operand):
+	 * (T) {t2, oovo, ovvv}; (fT) set 0 = L {l2, i6, i7}, set 1 = R {t2, i2, i3} */
+	const int nset_ = ft ? 2 : 1;
+	int famT[2] = {3, 3}, famO[2] = {4, 4}, famV[2] = {6, 6};
+	if (ft) {
+		famT[0] = 4; famO[0] = 8; famV[0] = 9;
+		famT[1] = 3; famO[1] = 6; famV[1] = 7;
+	}
+	const int KC = pt_kc(form), GB = pt_group_bytes(form);
+	const int nblk = restricted ? 2 : 4;
+	const double t_prep = now_s();
+
+	/* block geometry [R slabs][K kappa][PL plane] and element offsets,
+	 * src/pt.c:460-467 (restricted) and :487-512 (unrestricted) */
+	struct Geo {
+		int64_t R, K, PL, off;
+	};
+	auto geo = [&](int fam, int blk) {
+		const int64_t PA = va * (va - 1) / 2, PB = vb * (vb - 1) / 2;
+		Geo g[4];
+		if (fam == 0) {        /* t2-like [o][o][v][v] */
+			g[0] = {oa, oa, va * va, 0};
+			g[1] = {oa, ob, va * vb, 0};
+			g[2] = {ob, ob, vb * vb, 0};
+			g[3] = {ob, oa, vb * va, 0};
+		} else {               /* ovvv [o][v][v][v], same spin packed */
+			g[0] = {oa, va, PA, 0};
+			g[1] = {oa, vb, va * vb, 0};
+			g[2] = {ob, vb, PB, 0};
+			g[3] = {ob, va, vb * va, 0};
+		}
+		for (int b = 1; b < 4; ++b)
+			g[b].off = g[b - 1].off + g[b - 1].R * g[b - 1].K * g[b - 1].PL;
+		return g[blk];
+	};
+
+	/* F16X3 operand scales: powers of two bringing each family's max to
+	 * about 2^14 (amplitudes, integrals) */
+	for (int s = 0; s < nset_; ++s) {
+		double sa = 1, si = 1;
+		if (form == PT_FORM_F16X3) {
+			const double *fa[1] = {(const double *)arr[famT[s]]};
+			const double *fi[2] = {(const double *)arr[famO[s]],
+			    (const double *)arr[famV[s]]};
+			int64_t na_[1] = {n[famT[s]]}, ni_[2] = {n[famO[s]], n[famV[s]]};
+			PFAIL(family_scale(pl, fa, na_, 1, &sa));
+			PFAIL(family_scale(pl, fi, ni_, 2, &si));
+		}
+		pl->scale[2 * s] = sa;
+		pl->scale[2 * s + 1] = si;
+	}
+	/* S operands stay in the API layout (hi|lo pairs for the split forms) */
+	const void *sT[2] = {nullptr, nullptr}, *sO[2] = {nullptr, nullptr};
+	for (int s = 0; s < nset_; ++s) {
+		const int fam2[2] = {famT[s], famO[s]};
+		const double sc2[2] = {pl->scale[2 * s], pl->scale[2 * s + 1]};
+		const void *out2[2];
+		for (int q = 0; q < 2; ++q) {
+			if (!split) {
+				out2[q] = arr[fam2[q]];
+				continue;
 			}
-			double sa = 1, si = 1;
-			if (form == PT_FORM_F16X3) {
-				const double *fa[1] = {(const double *)arr[amp]};
-				const double *fi[2] = {(const double *)arr[i1],
-				    (const double *)arr[i2]};
-				int64_t na_[1] = {n[amp]}, ni_[2] = {n[i1], n[i2]};
-				PFAIL(family_scale(pl, fa, na_, 1, &sa));
-				PFAIL(family_scale(pl, fi, ni_, 2, &si));
-			}
-			pl->scale[2 * s] = sa;
-			pl->scale[2 * s + 1] = si;
-			const int fam[3] = {amp, i1, i2};
-			const double fsc[3] = {sa, si, si};
-			for (int q = 0; q < 3; ++q) {
+			void *dst;
+			PFAIL(dev_alloc(&dst, sizeof(uint32_t) * (size_t)n[fam2[q]]));
+			pl->owned.push_back(dst);
+			PFAIL(pt_launch_split((const double *)arr[fam2[q]],
+			    (uint32_t *)dst, n[fam2[q]], form == PT_FORM_F16X3, sc2[q],
+			    pl->stream) == 0 || (set_err("split launch failed"), false));
+			out2[q] = dst;
+		}
+		sT[s] = out2[0];
+		sO[s] = out2[1];
+	}
+	/* Q operands: packed copies of every amplitude and ovvv block */
+	Blocks qT[2], qV[2];
+	for (int s = 0; s < nset_; ++s) {
+		for (int fam = 0; fam < 2; ++fam) {
+			const int api = fam == 0 ? famT[s] : famV[s];
+			const double sc = pl->scale[2 * s + (fam == 0 ? 0 : 1)];
+			Blocks &dstb = fam == 0 ? qT[s] : qV[s];
+			for (int b = 0; b < nblk; ++b) {
+				const Geo g = geo(fam, b);
+				const int64_t nkb = (g.K + KC - 1) / KC;
 				void *dst;
-				PFAIL(dev_alloc(&dst, sizeof(uint32_t) * (size_t)n[fam[q]]));
+				PFAIL(dev_alloc(&dst, (size_t)(g.R * nkb * g.PL) * GB));
 				pl->owned.push_back(dst);
-				PFAIL(pt_launch_split((const double *)arr[fam[q]],
-				    (uint32_t *)dst, n[fam[q]],
-				    form == PT_FORM_F16X3, fsc[q], pl->stream) == 0 ||
-				    (set_err("split launch failed"), false));
-				oparr[fam[q]] = dst;
+				const char *src = (const char *)arr[api] + esz * (size_t)g.off;
+				PFAIL(pt_launch_pack(src, dst, g.R, g.K, g.PL, KC, form, sc,
+				    pl->stream) == 0 || (set_err("pack launch failed"), false));
+				dstb.p[b] = (const char *)dst;
+			}
+			if (restricted) {
+				dstb.p[2] = dstb.p[0];
+				dstb.p[3] = dstb.p[1];
 			}
 		}
-		PFAIL(hipStreamSynchronize(pl->stream) == hipSuccess);
-		pl->prep_s = now_s() - t0;
 	}
-	(void)opidx;
-	(void)nop;
+	PFAIL(hipStreamSynchronize(pl->stream) == hipSuccess ||
+	    (set_err("operand preparation failed"), false));
+	pl->prep_s = now_s() - t_prep;
 
 	/* block pointers (element offsets as in src/pt.c:460-467, 487-512) */
 	auto blk4 = [&](const void *base, int fam, size_t el) {
@@ -461,20 +520,12 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	ov2(arr[2], esz, &uA, &uB);
 	/* epilogue T2-like (Z uses t2 for (T), l2 for (fT)) and oovv */
 	Blocks zt = blk4(arr[ft ? 4 : 3], 0, esz);
-	Blocks gv = blk4(arr[ft ? 5 : 5], 0, esz);
-	/* operand families per W set: [set][T, O, V] */
-	Blocks opT[2], opO[2], opV[2];
-	if (!ft) {
-		opT[0] = blk4(oparr[3], 0, osz);
-		opO[0] = blk4(oparr[4], 1, osz);
-		opV[0] = blk4(oparr[6], 2, osz);
-	} else {
-		opT[0] = blk4(oparr[4], 0, osz); /* L: l2, i6, i7 */
-		opO[0] = blk4(oparr[8], 1, osz);
-		opV[0] = blk4(oparr[9], 2, osz);
-		opT[1] = blk4(oparr[3], 0, osz); /* R: t2, i2, i3 */
-		opO[1] = blk4(oparr[6], 1, osz);
-		opV[1] = blk4(oparr[7], 2, osz);
+	Blocks gv = blk4(arr[5], 0, esz);
+	/* S operand blocks per W set */
+	Blocks opT[2], opO[2];
+	for (int s = 0; s < nset_; ++s) {
+		opT[s] = blk4(sT[s], 0, osz);
+		opO[s] = blk4(sO[s], 1, osz);
 	}
 	const int nset = ft ? 2 : 1;
 	const double fac = restricted ? 2.0 : 1.0;
@@ -503,15 +554,19 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 		int64_t o1 = sp == 0 ? oa : ob, v1 = sp == 0 ? va : vb;
 		int64_t o2 = sp == 0 ? ob : oa, v2 = sp == 0 ? vb : va;
 		for (int s = 0; s < nset; ++s) {
-			c.L.buf[9 * s + 0] = opT[s].p[same];
-			c.L.buf[9 * s + 1] = opT[s].p[mixB];
-			c.L.buf[9 * s + 2] = opT[s].p[mixC];
-			c.L.buf[9 * s + 3] = opO[s].p[same];
-			c.L.buf[9 * s + 4] = opO[s].p[mixB];
-			c.L.buf[9 * s + 5] = opO[s].p[mixC];
-			c.L.buf[9 * s + 6] = opV[s].p[same];
-			c.L.buf[9 * s + 7] = opV[s].p[mixB];
-			c.L.buf[9 * s + 8] = opV[s].p[mixC];
+			const int sl = 12 * s;
+			c.L.buf[sl + 0] = opT[s].p[same];
+			c.L.buf[sl + 1] = opT[s].p[mixB];
+			c.L.buf[sl + 2] = opT[s].p[mixC];
+			c.L.buf[sl + 3] = opO[s].p[same];
+			c.L.buf[sl + 4] = opO[s].p[mixB];
+			c.L.buf[sl + 5] = opO[s].p[mixC];
+			c.L.buf[sl + 6] = qV[s].p[same];
+			c.L.buf[sl + 7] = qV[s].p[mixB];
+			c.L.buf[sl + 8] = qV[s].p[mixC];
+			c.L.buf[sl + 9] = qT[s].p[same];
+			c.L.buf[sl + 10] = qT[s].p[mixB];
+			c.L.buf[sl + 11] = qT[s].p[mixC];
 			c.L.inv_scale[s] = 1.0 / (pl->scale[2 * s] * pl->scale[2 * s + 1]);
 		}
 		pt_epi &e = c.L.epi;
@@ -760,6 +815,19 @@ libpt_amd_synth(const void *spec_, int tensor, void *dst, int on_device,
 		else
 			((double *)dst)[t] = x;
 	}
+	return 0;
+}
+
+extern "C" int
+libpt_amd_copy_kernel(void *dst, const void *src, int64_t n)
+{
+	if (!dev_init())
+		return -1;
+	if (pt_launch_copy((const double *)src, (double *)dst, n, nullptr) != 0) {
+		set_err("copy launch failed");
+		return -1;
+	}
+	HCHK(hipDeviceSynchronize(), -1);
 	return 0;
 }
 

@@ -78,28 +78,35 @@ template <> struct Form<PT_FORM_BF16X3> {
 template <> struct Form<PT_FORM_F16X3> : Form<PT_FORM_BF16X3> {};
 
 /*
- * LDS: S rows [32][KCP] and Q rows [32 u][QP w][KCP] per plane.  Q rows are
- * skewed (QP = 33) so 16 consecutive u at one w fall in different banks;
- * FP64/FP32 rows are padded by one element, split rows (64 bytes of
- * halves) have their 16-byte chunks XOR-swizzled instead.
+ * LDS (bytes): per plane, S rows [32][64 B] and Q rows [32 u][33 w][64 B]
+ * (one row = one kappa chunk of one (u, w) pair: 8 FP64, 16 FP32 or 32
+ * halves).  Q rows are skewed (33 w per u) so 16 consecutive u at one w
+ * land in different banks, and the four 16-byte chunks of each row are
+ * XOR-swizzled by (row / 4) % 4.  Split forms keep hi and lo halves in two
+ * planes.
  */
 template <int F> struct Lds {
 	typedef Form<F> FM;
-	static constexpr int KCP = FM::SPLIT ? FM::KC : FM::KC + 1;
 	static constexpr int QP = 33;
-	static constexpr int S_PLANE = 32 * KCP;
-	static constexpr int Q_PLANE = 32 * QP * KCP;
-	static constexpr int STAGE_BYTES =
-	    FM::NP * (S_PLANE + Q_PLANE) * (int)sizeof(typename FM::L);
+	static constexpr int ROWB = 64;
+	static constexpr int S_PB = 32 * ROWB;        /* S plane bytes */
+	static constexpr int Q_PB = 32 * QP * ROWB;   /* Q plane bytes */
+	static constexpr int STAGE_BYTES = FM::NP * (S_PB + Q_PB);
+	/* FP64/FP32 double-buffer their stages; split forms use one */
+	static constexpr int NSTAGE = FM::SPLIT ? 1 : 2;
 	static constexpr int X_BYTES = 4 * 32 * 64 * (int)sizeof(typename FM::AE);
-	static constexpr int BYTES =
-	    STAGE_BYTES > X_BYTES ? STAGE_BYTES : X_BYTES;
-	static __device__ __forceinline__ int off(int row, int k)
+	static constexpr int BYTES = NSTAGE * STAGE_BYTES > X_BYTES ?
+	    NSTAGE * STAGE_BYTES : X_BYTES;
+	/* byte offset of 16-byte chunk `ch` of row `row` */
+	static __device__ __forceinline__ int off(int row, int ch)
 	{
-		if constexpr (FM::SPLIT)
-			return row * 32 + (((k >> 3) ^ ((row >> 2) & 3)) << 3) + (k & 7);
-		else
-			return row * KCP + k;
+		return row * ROWB + ((ch ^ ((row >> 2) & 3)) << 4);
+	}
+	/* byte offset of element k (of size ES) of row `row` */
+	template <int ES>
+	static __device__ __forceinline__ int eoff(int row, int k)
+	{
+		return off(row, (k * ES) >> 4) + ((k * ES) & 15);
 	}
 };
 
@@ -202,6 +209,7 @@ struct Step {
 	int64_t s_st, q_k, q_u, q_w;
 	int role, set, K, neg, packed, ufast;
 	int s0, sn, sext, u0, uext, w0, wext, qwe, qsh;
+	int lo_mask, hi_sh;  /* pair index split: fast index bits, shift */
 };
 
 __device__ __forceinline__ Step
@@ -235,186 +243,297 @@ make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
 		ur = PT_ROLE_A; wr = PT_ROLE_B;
 	}
 	s.qsh = s.qwe == 32 ? 5 : 4;
+	s.lo_mask = s.ufast ? 31 : s.qwe - 1;
+	s.hi_sh = s.ufast ? 5 : s.qsh;
 	s.q_u = T.q_st[ur];
 	s.q_w = T.q_st[wr];
 	s.S = (const char *)s.S; /* offsets applied by the caller's type */
 	return s;
 }
 
-/* pairs (u, w) of this thread for a step: pair index tid + 256 m */
+/* (u, w) of pair index pidx: the contiguous index of Q varies fastest */
 __device__ __forceinline__ void
 pair_uw(const Step &s, int pidx, int &u, int &w)
 {
-	if (s.ufast) {
-		u = pidx & 31;
-		w = pidx >> 5;
-	} else {
-		w = pidx & (s.qwe - 1);
-		u = pidx >> s.qsh;
-	}
+	const int lo = pidx & s.lo_mask, hi = pidx >> s.hi_sh;
+	u = s.ufast ? lo : hi;
+	w = s.ufast ? hi : lo;
 }
 
-template <int F, int NPR, int SEPT>
+/* raw buffer loads: offsets at or beyond the descriptor's byte count
+ * return zero, which handles tile edges, invalid pairs and the kappa tail
+ * without branches */
+template <class G>
+__device__ __forceinline__ G
+bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+	if constexpr (sizeof(G) == 8)
+		return __builtin_bit_cast(G,
+		    __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+	else
+		return __builtin_bit_cast(G,
+		    __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t
+brsrc(const void *base, int64_t bytes)
+{
+	const int nrec = bytes <= 0 ? 0 : bytes > 0x7fffffff ? 0x7fffffff :
+	    (int)bytes;
+	return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, nrec,
+	    0x00020000);
+}
+
+#define PT_OOR 0x80000000u  /* an offset every descriptor rejects */
+
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+/*
+ * One staging step = (term, kappa chunk).  S: SEPT consecutive kappa of one
+ * row per thread, read in the API layout.  Q: 16-byte pieces of the packed
+ * copy; thread tid always moves chunk (tid % NV) of pairs
+ * tid / NV + n * (256 / NV), so one wave load covers 16 (or 8) whole pair
+ * groups of consecutive memory.
+ */
+template <int F, int NPC, int SEPT>
 __device__ __forceinline__ void
 load_step(const Step &s, int64_t soff, int64_t qoff, int kc, int tid,
-    typename Form<F>::G (&qr)[NPR][Form<F>::KC],
-    typename Form<F>::G (&sr)[SEPT])
+    v4u (&qr)[NPC], typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
 	constexpr int KC = Form<F>::KC;
-	const G *S = (const G *)s.S + soff;
-	const G *Q = (const G *)s.Q + qoff;
-	/* S: thread owns SEPT consecutive kappa of one row */
+	constexpr int ES = (int)sizeof(G);
+	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
+	constexpr int NV = GB / 16, LNV = NV == 8 ? 3 : 2;
 	{
 		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
 		const int sg = s.s0 + row;
 		const bool rv = row < s.sn && sg < s.sext;
-		const G *sp = S + (int64_t)(rv ? sg : 0) * s.s_st;
+		const G *sb = (const G *)s.S + soff;
+		const __amdgpu_buffer_rsrc_t rs = brsrc(sb,
+		    (int64_t)s.sext * s.s_st * ES);
 #pragma unroll
 		for (int e = 0; e < SEPT; ++e) {
 			const int kg = kc + k0 + e;
-			sr[e] = (rv && kg < s.K) ? neg_if(sp[kg], s.neg) : G(0);
+			const uint32_t o = (rv && kg < s.K) ?
+			    (uint32_t)(((int64_t)sg * s.s_st + kg) * ES) : PT_OOR;
+			sr[e] = neg_if(bload<G>(rs, o), s.neg);
 		}
 	}
-	const int npr = (32 * s.qwe) >> 8;
+	/* packed Q: descriptor over the kappa block of this chunk */
+	const char *qb = (const char *)s.Q + qoff + (int64_t)(kc / KC) * s.q_k * GB;
+	const __amdgpu_buffer_rsrc_t rq = brsrc(qb, s.q_k * GB);
+	const int ch = tid & (NV - 1);
+	const int npc = ((32 * s.qwe) << LNV) >> 8;
 #pragma unroll
-	for (int m = 0; m < NPR; ++m) {
-		if (m >= npr)
-			break;
+	for (int n = 0; n < NPC; ++n) {
+		const int pair = (tid >> LNV) + n * (256 >> LNV);
 		int u, w;
-		pair_uw(s, tid + 256 * m, u, w);
+		pair_uw(s, pair, u, w);
 		const int ug = s.u0 + u, wg = s.w0 + w;
-		const bool v = ug < s.uext && wg < s.wext && (!s.packed || ug > wg);
-		int64_t base = 0;
-		if (v)
-			base = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
-			    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
-		const G *qp = Q + base + (int64_t)kc * s.q_k;
-		const int kmax = v ? s.K - kc : 0;
-#pragma unroll
-		for (int kk = 0; kk < KC; ++kk)
-			qr[m][kk] = kk < kmax ? qp[(int64_t)kk * s.q_k] : G(0);
+		const bool v = n < npc && ug < s.uext && wg < s.wext &&
+		    (!s.packed || ug > wg);
+		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
+		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
+		const uint32_t o = v ? (uint32_t)(pidx * GB + ch * 16) : PT_OOR;
+		qr[n] = __builtin_bit_cast(v4u,
+		    __builtin_amdgcn_raw_buffer_load_b128(rq, (int)o, 0, 0));
 	}
 }
 
-template <int F, int NPR, int SEPT>
+template <int F, int NPC, int SEPT>
 __device__ __forceinline__ void
-store_step(const Step &s, int tid, typename Form<F>::L *sS,
-    typename Form<F>::L *sQ, const typename Form<F>::G (&qr)[NPR][Form<F>::KC],
+store_step(const Step &s, int tid, char *sS, char *sQ, const v4u (&qr)[NPC],
     const typename Form<F>::G (&sr)[SEPT])
 {
 	typedef Form<F> FM;
 	typedef Lds<F> LD;
 	constexpr int KC = FM::KC;
+	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
+	constexpr int NV = GB / 16, LNV = NV == 8 ? 3 : 2;
 	{
 		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
-#pragma unroll
-		for (int e = 0; e < SEPT; ++e) {
-			const int o = LD::off(row, k0 + e);
-			if constexpr (FM::SPLIT) {
-				sS[o] = (uint16_t)(sr[e] & 0xffffu);
-				sS[LD::S_PLANE + o] = (uint16_t)(sr[e] >> 16);
-			} else {
-				sS[o] = sr[e];
-			}
-		}
-	}
-	const int npr = (32 * s.qwe) >> 8;
-#pragma unroll
-	for (int m = 0; m < NPR; ++m) {
-		if (m >= npr)
-			break;
-		int u, w;
-		pair_uw(s, tid + 256 * m, u, w);
-		const int row = u * LD::QP + w;
 		if constexpr (FM::SPLIT) {
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				v8u h, l;
-#pragma unroll
-				for (int e = 0; e < 8; ++e) {
-					h[e] = (uint16_t)(qr[m][8 * q + e] & 0xffffu);
-					l[e] = (uint16_t)(qr[m][8 * q + e] >> 16);
-				}
-				const int o = LD::off(row, 8 * q);
-				*(v8u *)(sQ + o) = h;
-				*(v8u *)(sQ + LD::Q_PLANE + o) = l;
-			}
+			/* SEPT = 4 kappa: 8 bytes of hi halves, 8 of lo halves */
+			uint32_t h01 = (sr[0] & 0xffffu) | (sr[1] << 16);
+			uint32_t h23 = (sr[2] & 0xffffu) | (sr[3] << 16);
+			uint32_t l01 = (sr[0] >> 16) | (sr[1] & 0xffff0000u);
+			uint32_t l23 = (sr[2] >> 16) | (sr[3] & 0xffff0000u);
+			const int o = LD::template eoff<2>(row, k0);
+			*(uint2 *)(sS + o) = make_uint2(h01, h23);
+			*(uint2 *)(sS + LD::S_PB + o) = make_uint2(l01, l23);
 		} else {
+			typedef typename FM::L L;
 #pragma unroll
-			for (int kk = 0; kk < KC; ++kk)
-				sQ[LD::off(row, kk)] = qr[m][kk];
+			for (int e = 0; e < SEPT; ++e)
+				*(L *)(sS + LD::template eoff<(int)sizeof(L)>(row,
+				    k0 + e)) = sr[e];
 		}
 	}
+	const int ch = tid & (NV - 1);
+	const int npc = ((32 * s.qwe) << LNV) >> 8;
+#pragma unroll
+	for (int n = 0; n < NPC; ++n) {
+		const int pair = (tid >> LNV) + n * (256 >> LNV);
+		int u, w;
+		pair_uw(s, pair, u, w);
+		const int row = u * LD::QP + w;
+		if (n < npc)
+			*(v4u *)(sQ + (ch >> 2) * LD::Q_PB + LD::off(row, ch & 3)) =
+			    qr[n];
+	}
+}
+
+/*
+ * Direct HBM -> LDS staging (FP64 / FP32).  The Q area of a stage is 66
+ * runs of 1 KB (16 rows of 64 bytes); one buffer_load_dwordx4 ... lds of a
+ * wave fills one run, lane l writing physical chunk l % 4 of row
+ * 16 run + l / 4.  Lane offsets depend only on the term, not on kappa, so
+ * they are computed once per term; each step only moves the descriptor.
+ */
+#define PT_QRUNS 66
+#define PT_RUNS_PER_WAVE 17
+
+__device__ __forceinline__ void
+qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_RUNS_PER_WAVE],
+    int gb)
+{
+#pragma unroll
+	for (int n = 0; n < PT_RUNS_PER_WAVE; ++n) {
+		const int run = wave + 4 * n;
+		const int r = 16 * run + (lane >> 2);
+		const int u = r / 33, w = r - 33 * u;
+		const int ug = s.u0 + u, wg = s.w0 + w;
+		const bool v = run < PT_QRUNS && w < s.qwe && ug < s.uext &&
+		    wg < s.wext && (!s.packed || ug > wg);
+		const int lch = (lane & 3) ^ ((r >> 2) & 3);
+		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
+		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
+		vo[n] = v ? (uint32_t)(pidx * gb + lch * 16) : PT_OOR;
+	}
+}
+
+template <int GB, int KC>
+__device__ __forceinline__ void
+issue_q(const Step &s, int64_t qoff, int kc, char *sQ, int wave,
+    const uint32_t (&vo)[PT_RUNS_PER_WAVE])
+{
+	const char *qb = (const char *)s.Q + qoff + (int64_t)(kc / KC) * s.q_k * GB;
+	const __amdgpu_buffer_rsrc_t rq = brsrc(qb, s.q_k * GB);
+#pragma unroll
+	for (int n = 0; n < PT_RUNS_PER_WAVE; ++n) {
+		const int run = wave + 4 * n;
+		if (run < PT_QRUNS)
+			__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
+			    (__attribute__((address_space(3))) void *)(sQ + 1024 * run),
+			    16, vo[n], 0, 0, 0);
+	}
+}
+
+__device__ __forceinline__ void
+wait_vm()
+{
+	__asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+/* S rows of one step: load SEPT kappa per thread, store into a stage */
+template <int F, int SEPT>
+__device__ __forceinline__ void
+load_s(const Step &s, int64_t soff, int kc, int tid,
+    typename Form<F>::G (&sr)[SEPT])
+{
+	typedef typename Form<F>::G G;
+	constexpr int KC = Form<F>::KC;
+	constexpr int ES = (int)sizeof(G);
+	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+	const int sg = s.s0 + row;
+	const bool rv = row < s.sn && sg < s.sext;
+	const G *sb = (const G *)s.S + soff;
+	const __amdgpu_buffer_rsrc_t rs = brsrc(sb, (int64_t)s.sext * s.s_st * ES);
+#pragma unroll
+	for (int e = 0; e < SEPT; ++e) {
+		const int kg = kc + k0 + e;
+		const uint32_t o = (rv && kg < s.K) ?
+		    (uint32_t)(((int64_t)sg * s.s_st + kg) * ES) : PT_OOR;
+		sr[e] = neg_if(bload<G>(rs, o), s.neg);
+	}
+}
+
+template <int F, int SEPT>
+__device__ __forceinline__ void
+store_s(int tid, char *sS, const typename Form<F>::G (&sr)[SEPT])
+{
+	typedef Form<F> FM;
+	typedef Lds<F> LD;
+	typedef typename FM::L L;
+	constexpr int KC = FM::KC;
+	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+#pragma unroll
+	for (int e = 0; e < SEPT; ++e)
+		*(L *)(sS + LD::template eoff<(int)sizeof(L)>(row, k0 + e)) = sr[e];
 }
 
 /* MFMA over one staged chunk into a fixed accumulator set */
 template <int F, int NCB>
 __device__ __forceinline__ void
-mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role,
-    const typename Form<F>::L *sS, const typename Form<F>::L *sQ, int wa,
-    int wb, int ln, int lg)
+mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
+    const char *sQ, int wa, int wb, int ln, int lg)
 {
 	typedef Form<F> FM;
 	typedef Lds<F> LD;
 	typedef typename FM::L L;
 	constexpr int KC = FM::KC, KS = FM::KS, QP = LD::QP;
 	constexpr int TCW = 16 * NCB;
+	constexpr int ES = (int)sizeof(L);
+#define LDE(base, row, k) (*(const L *)((base) + LD::template eoff<ES>((row), (k))))
 	if constexpr (!FM::SPLIT) {
+		/* one branch-free MFMA sequence for all roles (a single flow of
+		 * the accumulators keeps them in place); block x reads
+		 *   A row ra0 + (x % 16) ra1 + (x / 16) ra2 of pa,
+		 *   B row rb0 + (x % 16) rb1 + (x / 16) rb2 of pb.
+		 * role a: A = S[a],    B = Q[b][c]
+		 * role b: A = Q[a][c], B = S[b]
+		 * role c: A = Q[a][b], B = S[c]  (block x = 16 cb + b) */
+		const bool ra = role == PT_ROLE_A, rb = role == PT_ROLE_B;
+		const char *pa = ra ? sS : sQ, *pb = ra ? sQ : sS;
+		const int ra0 = ra ? 16 * wa + ln :
+		    (16 * wa + ln) * QP + (rb ? 0 : 16 * wb);
+		const int ra1 = ra ? 0 : 1, ra2 = rb ? 16 : 0;
+		const int rb0 = ra ? (16 * wb + ln) * QP : rb ? 16 * wb + ln : ln;
+		const int rb1 = ra ? 1 : 0, rb2 = rb ? 0 : 16;
 #pragma unroll
 		for (int ks = 0; ks < KC; ks += KS) {
 			const int kap = ks + lg;
-			if (role == PT_ROLE_A) {
-				const L av = sS[LD::off(16 * wa + ln, kap)];
-				const int r0 = (16 * wb + ln) * QP;
 #pragma unroll
-				for (int x = 0; x < TCW; ++x)
-					ac[x] = mma(av, sQ[LD::off(r0 + x, kap)], ac[x]);
-			} else if (role == PT_ROLE_B) {
-				const L bv = sS[LD::off(16 * wb + ln, kap)];
-				const int r0 = (16 * wa + ln) * QP;
-#pragma unroll
-				for (int x = 0; x < TCW; ++x)
-					ac[x] = mma(sQ[LD::off(r0 + x, kap)], bv, ac[x]);
-			} else {
-				L bv[NCB];
-#pragma unroll
-				for (int cb = 0; cb < NCB; ++cb)
-					bv[cb] = sS[LD::off(16 * cb + ln, kap)];
-				const int r0 = (16 * wa + ln) * QP + 16 * wb;
-#pragma unroll
-				for (int b = 0; b < 16; ++b) {
-					const L av = sQ[LD::off(r0 + b, kap)];
-#pragma unroll
-					for (int cb = 0; cb < NCB; ++cb)
-						ac[16 * cb + b] = mma(av, bv[cb], ac[16 * cb + b]);
-				}
+			for (int x = 0; x < TCW; ++x) {
+				const int rA = ra0 + (x & 15) * ra1 + (x >> 4) * ra2;
+				const int rB = rb0 + (x & 15) * rb1 + (x >> 4) * rb2;
+				ac[x] = mma(LDE(pa, rA, kap), LDE(pb, rB, kap), ac[x]);
 			}
 		}
 	} else {
 #define LD8(base, plane, row) \
-	(*(const v8u *)((base) + (plane) + (row) * 32 + \
-	    ((lg ^ (((row) >> 2) & 3)) << 3)))
+	(*(const v8u *)((base) + (plane) + LD::off((row), lg)))
 		if (role == PT_ROLE_A) {
 			const int rs = 16 * wa + ln;
-			const v8u ah = LD8(sS, 0, rs), al = LD8(sS, LD::S_PLANE, rs);
+			const v8u ah = LD8(sS, 0, rs), al = LD8(sS, LD::S_PB, rs);
 			const int r0 = (16 * wb + ln) * QP;
 #pragma unroll
 			for (int x = 0; x < TCW; ++x) {
 				const v8u bh = LD8(sQ, 0, r0 + x);
-				const v8u bl = LD8(sQ, LD::Q_PLANE, r0 + x);
+				const v8u bl = LD8(sQ, LD::Q_PB, r0 + x);
 				ac[x] = mma8<F>(al, bh, ac[x]);
 				ac[x] = mma8<F>(ah, bl, ac[x]);
 				ac[x] = mma8<F>(ah, bh, ac[x]);
 			}
 		} else if (role == PT_ROLE_B) {
 			const int rs = 16 * wb + ln;
-			const v8u bh = LD8(sS, 0, rs), bl = LD8(sS, LD::S_PLANE, rs);
+			const v8u bh = LD8(sS, 0, rs), bl = LD8(sS, LD::S_PB, rs);
 			const int r0 = (16 * wa + ln) * QP;
 #pragma unroll
 			for (int x = 0; x < TCW; ++x) {
 				const v8u ah = LD8(sQ, 0, r0 + x);
-				const v8u al = LD8(sQ, LD::Q_PLANE, r0 + x);
+				const v8u al = LD8(sQ, LD::Q_PB, r0 + x);
 				ac[x] = mma8<F>(al, bh, ac[x]);
 				ac[x] = mma8<F>(ah, bl, ac[x]);
 				ac[x] = mma8<F>(ah, bh, ac[x]);
@@ -424,13 +543,13 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role,
 #pragma unroll
 			for (int cb = 0; cb < NCB; ++cb) {
 				bh[cb] = LD8(sS, 0, 16 * cb + ln);
-				bl[cb] = LD8(sS, LD::S_PLANE, 16 * cb + ln);
+				bl[cb] = LD8(sS, LD::S_PB, 16 * cb + ln);
 			}
 			const int r0 = (16 * wa + ln) * QP + 16 * wb;
 #pragma unroll
 			for (int b = 0; b < 16; ++b) {
 				const v8u ah = LD8(sQ, 0, r0 + b);
-				const v8u al = LD8(sQ, LD::Q_PLANE, r0 + b);
+				const v8u al = LD8(sQ, LD::Q_PB, r0 + b);
 #pragma unroll
 				for (int cb = 0; cb < NCB; ++cb) {
 					const int x = 16 * cb + b;
@@ -442,12 +561,18 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role,
 		}
 #undef LD8
 	}
+#undef LDE
 }
 
 /* -------------------------------------------------------------- kernel */
 
+/* workgroups per CU the register allocation targets */
+template <int F, int NSET> struct Occ {
+	static constexpr int WG_PER_CU = 1;
+};
+
 template <int F, int KIND, int NSET>
-__global__ __launch_bounds__(PT_THREADS, 1) void
+__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
 k_tiles(pt_launch p)
 {
 	typedef Form<F> FM;
@@ -460,13 +585,14 @@ k_tiles(pt_launch p)
 	constexpr int KC = FM::KC;
 	constexpr int NCB = (F == PT_FORM_FP32 && NSET == 1) ? 2 : 1; /* pt_tcw */
 	constexpr int TCW = 16 * NCB;
-	constexpr int NPR = 4;                     /* (u, w) pairs per thread */
+	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
+	constexpr int NPC = 32 * 32 * (GB / 16) / PT_THREADS; /* Q pieces/thread */
 	constexpr int SEPT = 32 * KC / PT_THREADS; /* S elements per thread */
 
 	__shared__ __attribute__((aligned(16))) char smem[LD::BYTES];
 	__shared__ double red[4];
-	L *sS = (L *)smem;
-	L *sQ = sS + FM::NP * LD::S_PLANE;
+	char *sS = smem;
+	char *sQ = smem + FM::NP * LD::S_PB;
 	AE *sX = (AE *)smem;
 
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -501,65 +627,124 @@ k_tiles(pt_launch p)
 
 	/* steps = (term, kappa chunk) over all terms; phase 1 first */
 	const int nterm = p.nterm1 + p.nterm2;
-	G qr[NPR][KC], sr[SEPT];
 	int t = 0, kc = 0;
 	Step cur = make_step(p, p.terms[0], i, j, k, A0, B0, C0, TCW);
-	int64_t soff = 0, qoff = 0;
+	int64_t soff, qoff;
 	{
 		const pt_term &T = p.terms[0];
 		soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
 		qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
 	}
-	load_step<F, NPR, SEPT>(cur, soff, qoff, 0, tid, qr, sr);
+	G sr[SEPT];
 
-	while (t < nterm) {
-		/* phase boundary: [a][b]@c -> [a][c]@b, per register r */
-		if (t == p.nterm1 && kc == 0) {
+	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
+	auto transpose = [&](AE *scratch) {
 #pragma unroll
-			for (int s = 0; s < NSET; ++s)
+		for (int s = 0; s < NSET; ++s)
 #pragma unroll
-			for (int cb = 0; cb < NCB; ++cb)
+		for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-			for (int r = 0; r < 4; ++r) {
-				AE *X = sX + wave * (4 * 16 * 17);
-				__syncthreads();
+		for (int r = 0; r < 4; ++r) {
+			AE *X = scratch + wave * (4 * 16 * 17);
+			__syncthreads();
 #pragma unroll
-				for (int c = 0; c < 16; ++c)
-					X[(lg * 16 + ln) * 17 + c] = acc[s][16 * cb + c][r];
-				__syncthreads();
+			for (int c = 0; c < 16; ++c)
+				X[(lg * 16 + ln) * 17 + c] = acc[s][16 * cb + c][r];
+			__syncthreads();
 #pragma unroll
-				for (int b = 0; b < 16; ++b)
-					acc[s][16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
-			}
+			for (int b = 0; b < 16; ++b)
+				acc[s][16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
 		}
 		__syncthreads();
-		store_step<F, NPR, SEPT>(cur, tid, sS, sQ, qr, sr);
+	};
+
+	if constexpr (!FM::SPLIT) {
+		/* double-buffered stages, Q streamed straight into LDS */
+		constexpr int STAGE = LD::STAGE_BYTES;
+		const int wv = __builtin_amdgcn_readfirstlane(wave);
+		uint32_t vo[PT_RUNS_PER_WAVE];
+		qtab(cur, wv, lane, vo, GB);
+		load_s<F, SEPT>(cur, soff, 0, tid, sr);
+		issue_q<GB, KC>(cur, qoff, 0, smem + LD::S_PB, wv, vo);
+		wait_vm();
+		store_s<F, SEPT>(tid, smem, sr);
 		__syncthreads();
-		/* prefetch the next step while this one computes */
-		const int role = cur.role, set = cur.set;
-		int tn = t, kn = kc + KC;
-		if (kn >= cur.K) {
-			tn = t + 1;
-			kn = 0;
-		}
-		if (tn < nterm) {
-			if (tn != t) {
-				const pt_term &T = p.terms[tn];
-				cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
-				soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
-				qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+		int buf = 0;
+		while (t < nterm) {
+			const int role = cur.role, set = cur.set;
+			int tn = t, kn = kc + KC;
+			if (kn >= cur.K) {
+				tn = t + 1;
+				kn = 0;
 			}
-			load_step<F, NPR, SEPT>(cur, soff, qoff, kn, tid, qr, sr);
+			char *st_cur = smem + buf * STAGE;
+			char *st_nxt = smem + (buf ^ 1) * STAGE;
+			if (tn < nterm) {
+				if (tn != t) {
+					const pt_term &T = p.terms[tn];
+					cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
+					soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
+					qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+					qtab(cur, wv, lane, vo, GB);
+				}
+				load_s<F, SEPT>(cur, soff, kn, tid, sr);
+				issue_q<GB, KC>(cur, qoff, kn, st_nxt + LD::S_PB, wv, vo);
+			}
+			if (act) {
+				if (NSET == 1 || set == 0)
+					mma_step<F, NCB>(acc[0], role, st_cur, st_cur + LD::S_PB,
+					    wa, wb, ln, lg);
+				else
+					mma_step<F, NCB>(acc[NSET - 1], role, st_cur,
+					    st_cur + LD::S_PB, wa, wb, ln, lg);
+			}
+			if (tn < nterm) {
+				wait_vm();
+				store_s<F, SEPT>(tid, st_nxt, sr);
+			}
+			__syncthreads();
+			/* phase boundary: the stage just consumed is scratch */
+			if (tn == p.nterm1 && kn == 0 && tn < nterm)
+				transpose((AE *)st_cur);
+			t = tn;
+			kc = kn;
+			buf ^= 1;
 		}
-		if (act) {
-			if (NSET == 1 || set == 0)
-				mma_step<F, NCB>(acc[0], role, sS, sQ, wa, wb, ln, lg);
-			else
-				mma_step<F, NCB>(acc[NSET - 1], role, sS, sQ, wa, wb, ln,
-				    lg);
+	} else {
+		v4u qr[NPC];
+		load_step<F, NPC, SEPT>(cur, soff, qoff, 0, tid, qr, sr);
+		while (t < nterm) {
+			if (t == p.nterm1 && kc == 0)
+				transpose(sX);
+			__syncthreads();
+			store_step<F, NPC, SEPT>(cur, tid, sS, sQ, qr, sr);
+			__syncthreads();
+			/* prefetch the next step while this one computes */
+			const int role = cur.role, set = cur.set;
+			int tn = t, kn = kc + KC;
+			if (kn >= cur.K) {
+				tn = t + 1;
+				kn = 0;
+			}
+			if (tn < nterm) {
+				if (tn != t) {
+					const pt_term &T = p.terms[tn];
+					cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
+					soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
+					qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+				}
+				load_step<F, NPC, SEPT>(cur, soff, qoff, kn, tid, qr, sr);
+			}
+			if (act) {
+				if (NSET == 1 || set == 0)
+					mma_step<F, NCB>(acc[0], role, sS, sQ, wa, wb, ln, lg);
+				else
+					mma_step<F, NCB>(acc[NSET - 1], role, sS, sQ, wa, wb,
+					    ln, lg);
+			}
+			t = tn;
+			kc = kn;
 		}
-		t = tn;
-		kc = kn;
 	}
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
@@ -663,6 +848,61 @@ k_segsum(const double *x, int64_t nseg, int64_t seglen, double *out)
 	out[s] = acc - c;
 }
 
+__device__ __forceinline__ void
+split2(double x, int fp16, uint16_t &hb, uint16_t &lb)
+{
+	if (fp16) {
+		_Float16 h = (_Float16)x;
+		_Float16 l = (_Float16)(x - (double)h);
+		hb = __builtin_bit_cast(uint16_t, h);
+		lb = __builtin_bit_cast(uint16_t, l);
+	} else {
+		__bf16 h = (__bf16)(float)x;
+		__bf16 l = (__bf16)(float)(x - (double)(float)h);
+		hb = __builtin_bit_cast(uint16_t, h);
+		lb = __builtin_bit_cast(uint16_t, l);
+	}
+}
+
+/* [R][K][PL] -> [R][nkb][PL][KC] (FP64/FP32) or [R][nkb][PL][hi KC][lo KC]
+ * (split, of x * scale); kappa >= K is zero.  One thread per group: reads
+ * coalesce over PL, each group is written contiguously. */
+__global__ void
+k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t PL, int KC,
+    int mode, double scale)
+{
+	const int64_t nkb = (K + KC - 1) / KC;
+	const int64_t total = R * nkb * PL;
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	    t < total; t += (int64_t)gridDim.x * blockDim.x) {
+		const int64_t pl = t % PL, rk = t / PL, kb = rk % nkb, r = rk / nkb;
+		const int64_t k0 = kb * KC;
+		const int64_t sbase = (r * K + k0) * PL + pl;
+		if (mode == PT_FORM_FP64) {
+			const double *sp = (const double *)src + sbase;
+			double *dp = (double *)dst + t * KC;
+			for (int kk = 0; kk < KC; ++kk)
+				dp[kk] = k0 + kk < K ? sp[(int64_t)kk * PL] : 0.0;
+		} else if (mode == PT_FORM_FP32) {
+			const float *sp = (const float *)src + sbase;
+			float *dp = (float *)dst + t * KC;
+			for (int kk = 0; kk < KC; ++kk)
+				dp[kk] = k0 + kk < K ? sp[(int64_t)kk * PL] : 0.0f;
+		} else {
+			const double *sp = (const double *)src + sbase;
+			uint16_t *dp = (uint16_t *)dst + t * 2 * KC;
+			for (int kk = 0; kk < KC; ++kk) {
+				uint16_t hb = 0, lb = 0;
+				if (k0 + kk < K)
+					split2(sp[(int64_t)kk * PL] * scale,
+					    mode == PT_FORM_F16X3, hb, lb);
+				dp[kk] = hb;
+				dp[KC + kk] = lb;
+			}
+		}
+	}
+}
+
 /* hi + lo split of (x * scale) into bf16 or fp16 pairs */
 __global__ void
 k_split(const double *src, uint32_t *dst, int64_t n, int fp16, double scale)
@@ -704,6 +944,15 @@ k_absmax(const double *src, int64_t n, unsigned long long *out)
 	}
 	if (threadIdx.x == 0)
 		atomicMax(out, (unsigned long long)__double_as_longlong(sh[0]));
+}
+
+/* plain copy (traffic counter calibration) */
+__global__ void
+k_copy(const double *src, double *dst, int64_t n)
+{
+	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+	    t += (int64_t)gridDim.x * blockDim.x)
+		dst[t] = src[t];
 }
 
 __global__ void
@@ -786,6 +1035,19 @@ pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
 }
 
 extern "C" int
+pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t PL,
+    int KC, int mode, double scale, void *stream)
+{
+	const int64_t total = R * ((K + KC - 1) / KC) * PL;
+	if (total <= 0)
+		return 0;
+	int64_t nb = (total + 255) / 256;
+	k_pack<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0,
+	    (hipStream_t)stream>>>(src, dst, R, K, PL, KC, mode, scale);
+	return (int)hipGetLastError();
+}
+
+extern "C" int
 pt_launch_absmax(const double *src, int64_t n, double *out, void *stream)
 {
 	if (n <= 0)
@@ -793,6 +1055,15 @@ pt_launch_absmax(const double *src, int64_t n, double *out, void *stream)
 	int64_t nb = (n + 255) / 256;
 	k_absmax<<<(unsigned)(nb < 4096 ? nb : 4096), 256, 0,
 	    (hipStream_t)stream>>>(src, n, (unsigned long long *)out);
+	return (int)hipGetLastError();
+}
+
+extern "C" int
+pt_launch_copy(const double *src, double *dst, int64_t n, void *stream)
+{
+	if (n <= 0)
+		return 0;
+	k_copy<<<16384, 256, 0, (hipStream_t)stream>>>(src, dst, n);
 	return (int)hipGetLastError();
 }
 

@@ -7,7 +7,8 @@
  * :773-915 reuse the same tables on other operand buffers.
  *
  * Occupied slots: 0 = i, 1 = j, 2 = k.  Operand buffer slots (per W set s,
- * base 9*s): TA TB TC OA OB OC VA VB VC (same spin uses the A slots).
+ * base 12*s): TA TB TC OA OB OC in the API layout (S operands), VA VB VC
+ * and TA TB TC packed (Q operands); the same-spin case uses the A slots.
  */
 #include <string.h>
 
@@ -75,7 +76,8 @@ struct TermBuilder {
 void
 pt_terms_same(std::vector<pt_term> &out, int set, int64_t o, int64_t v)
 {
-	const int bT = 9 * set + 0, bO = 9 * set + 3, bV = 9 * set + 6;
+	const int bT = 12 * set + 0, bO = 12 * set + 3, bV = 12 * set + 6;
+	const int bTq = 12 * set + 9;
 	const int64_t P = v * (v - 1) / 2, vv = v * v;
 
 	/* ovvv: Y_pqr(x; y,z) = sum_d T[p,q,x,d] V[r,d,(y,z)],
@@ -103,15 +105,15 @@ pt_terms_same(std::vector<pt_term> &out, int set, int64_t o, int64_t v)
 		const int s = sg1[n];
 		/* +X(b,c,a): S = O[q,r][a][l], Q = T[p][l][b][c] */
 		out.push_back(TermBuilder(A, set, +s, (int)o)
-		    .S(bO, q, o * v * o, r, v * o, o).Q(bT, p, o * vv, vv)
+		    .S(bO, q, o * v * o, r, v * o, o).Q(bTq, p, o * vv, vv)
 		    .st(B, v).st(C, 1).done());
 		/* -X(a,c,b): S = O[q,r][b][l], Q = T[p][l][a][c] */
 		out.push_back(TermBuilder(B, set, -s, (int)o)
-		    .S(bO, q, o * v * o, r, v * o, o).Q(bT, p, o * vv, vv)
+		    .S(bO, q, o * v * o, r, v * o, o).Q(bTq, p, o * vv, vv)
 		    .st(A, v).st(C, 1).done());
 		/* -X(b,a,c): S = O[q,r][c][l], Q = T[p][l][b][a] */
 		out.push_back(TermBuilder(C, set, -s, (int)o)
-		    .S(bO, q, o * v * o, r, v * o, o).Q(bT, p, o * vv, vv)
+		    .S(bO, q, o * v * o, r, v * o, o).Q(bTq, p, o * vv, vv)
 		    .st(A, 1).st(B, v).done());
 	}
 }
@@ -120,10 +122,11 @@ void
 pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
     int64_t ob, int64_t vb)
 {
-	const int b0 = 9 * set;
+	const int b0 = 12 * set;
 	const int TA = b0 + 0, TB = b0 + 1, TC = b0 + 2;
 	const int OA = b0 + 3, OB = b0 + 4, OC = b0 + 5;
 	const int VA = b0 + 6, VB = b0 + 7, VC = b0 + 8;
+	const int TAq = b0 + 9, TBq = b0 + 10, TCq = b0 + 11;
 	const int64_t PA = va * (va - 1) / 2;
 	const int ka = (int)va, kb = (int)vb, koa = (int)oa, kob = (int)ob;
 
@@ -158,31 +161,31 @@ pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
 	/* :394-406: -sum_l TA[i,l,a,b] OC[k,j,c,l] + sum_l TA[j,l,a,b] OC[k,i,c,l] */
 	out.push_back(TermBuilder(C, set, -1, koa)
 	    .S(OC, K, oa * vb * oa, J, vb * oa, oa)
-	    .Q(TA, I, oa * va * va, va * va).st(A, va).st(B, 1).done());
+	    .Q(TAq, I, oa * va * va, va * va).st(A, va).st(B, 1).done());
 	out.push_back(TermBuilder(C, set, +1, koa)
 	    .S(OC, K, oa * vb * oa, I, vb * oa, oa)
-	    .Q(TA, J, oa * va * va, va * va).st(A, va).st(B, 1).done());
+	    .Q(TAq, J, oa * va * va, va * va).st(A, va).st(B, 1).done());
 	/* :408-414: -sum_l TB[i,l,a,c] OB[j,k,b,l] + sum_l TB[i,l,b,c] OB[j,k,a,l] */
 	out.push_back(TermBuilder(B, set, -1, kob)
 	    .S(OB, J, ob * va * ob, K, va * ob, ob)
-	    .Q(TB, I, ob * va * vb, va * vb).st(A, vb).st(C, 1).done());
+	    .Q(TBq, I, ob * va * vb, va * vb).st(A, vb).st(C, 1).done());
 	out.push_back(TermBuilder(A, set, +1, kob)
 	    .S(OB, J, ob * va * ob, K, va * ob, ob)
-	    .Q(TB, I, ob * va * vb, va * vb).st(B, vb).st(C, 1).done());
+	    .Q(TBq, I, ob * va * vb, va * vb).st(B, vb).st(C, 1).done());
 	/* :416-422: -sum_l TB[j,l,b,c] OB[i,k,a,l] + sum_l TB[j,l,a,c] OB[i,k,b,l] */
 	out.push_back(TermBuilder(A, set, -1, kob)
 	    .S(OB, I, ob * va * ob, K, va * ob, ob)
-	    .Q(TB, J, ob * va * vb, va * vb).st(B, vb).st(C, 1).done());
+	    .Q(TBq, J, ob * va * vb, va * vb).st(B, vb).st(C, 1).done());
 	out.push_back(TermBuilder(B, set, +1, kob)
 	    .S(OB, I, ob * va * ob, K, va * ob, ob)
-	    .Q(TB, J, ob * va * vb, va * vb).st(A, vb).st(C, 1).done());
+	    .Q(TBq, J, ob * va * vb, va * vb).st(A, vb).st(C, 1).done());
 	/* :424-432: -sum_l TC[k,l,c,a] OA[j,i,b,l] + sum_l TC[k,l,c,b] OA[j,i,a,l] */
 	out.push_back(TermBuilder(B, set, -1, koa)
 	    .S(OA, J, oa * va * oa, I, va * oa, oa)
-	    .Q(TC, K, oa * vb * va, vb * va).st(A, 1).st(C, va).done());
+	    .Q(TCq, K, oa * vb * va, vb * va).st(A, 1).st(C, va).done());
 	out.push_back(TermBuilder(A, set, +1, koa)
 	    .S(OA, J, oa * va * oa, I, va * oa, oa)
-	    .Q(TC, K, oa * vb * va, vb * va).st(B, 1).st(C, va).done());
+	    .Q(TCq, K, oa * vb * va, vb * va).st(B, 1).st(C, va).done());
 }
 
 int
