@@ -47,10 +47,10 @@ pt_group_bytes(int form)
 }
 
 /* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
- * TCW is 32 only where the accumulators of one wave stay within the AGPR
- * file: FP32 (T) (32 blocks x 4 VGPRs); FP64 blocks take 8 VGPRs, (fT)
- * holds two W sets, and the split forms need registers for 32-deep
- * operand prefetch, so those use 16. */
+ * TCW = 32 only for FP32 (T): its 32 accumulator blocks (4 registers each)
+ * and the operands of a step fit the 256 VGPRs.  FP64 blocks take 8
+ * registers, (fT) holds two W sets, and the split forms keep hi / lo
+ * operand pairs, so those use 16. */
 #define PT_TA 32
 #define PT_TB 32
 #define PT_THREADS 256
@@ -111,6 +111,8 @@ extern "C" {
 #endif
 /* kernel launchers (pt_kernel.hip); return hipError_t as int */
 int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
+/* FP64 tile kernels with two W sets (built in a separate unit) */
+int pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream);
 int pt_launch_reduce(const double *x, int64_t n, double *out, void *stream);
 int pt_launch_segsum(const double *x, int64_t nseg, int64_t seglen,
     double *out, void *stream);

@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pt_engine.h"
 #include "synth.h"
 
@@ -385,48 +387,66 @@ store_step(const Step &s, int tid, char *sS, char *sQ, const v4u (&qr)[NPC],
 }
 
 /*
- * Direct HBM -> LDS staging (FP64 / FP32).  The Q area of a stage is 66
- * runs of 1 KB (16 rows of 64 bytes); one buffer_load_dwordx4 ... lds of a
- * wave fills one run, lane l writing physical chunk l % 4 of row
- * 16 run + l / 4.  Lane offsets depend only on the term, not on kappa, so
- * they are computed once per term; each step only moves the descriptor.
+ * FP64 / FP32 stage (bytes):
+ *   S: 32 rows x 80 B (one kappa chunk of 64 B plus 16 B of padding, so the
+ *      16 rows read by one MFMA operand fall into distinct bank groups);
+ *   Q: 32 u x (32 w x 64 B + 16 B): piece (u, w, ch) at u * 2064 + w * 64
+ *      + 16 ch.  The 16 consecutive u of one operand read land in distinct
+ *      16-byte bank groups, and consecutive w are 64 B apart, so every
+ *      operand read of a step is one base register plus an immediate.
+ * Lane group lg of an MFMA takes 16-byte chunk lg of a row, i.e. kappa
+ * 2lg, 2lg+1 (FP64) or 4lg .. 4lg+3 (FP32), for both operands; one LDS
+ * read therefore feeds 2 (FP64) or 4 (FP32) MFMAs.
  */
-#define PT_QRUNS 66
-#define PT_RUNS_PER_WAVE 17
+struct Stg {
+	static constexpr int S_ROW = 80;
+	static constexpr int S_BYTES = 32 * S_ROW;
+	static constexpr int Q_USTR = 32 * 64 + 16;
+	static constexpr int Q_BYTES = 32 * Q_USTR;
+	static constexpr int STAGE = S_BYTES + Q_BYTES;
+	static constexpr int BYTES = 2 * STAGE;
+};
+
+/*
+ * Q is streamed HBM -> LDS without registers (buffer_load ... lds): one
+ * wave instruction fills 1 KB of LDS (lane l at +16 l), i.e. 16 w rows of
+ * one u.  The 64 runs (u = run % 32, w half = run / 32) are dealt to the
+ * waves round-robin.  Lane offsets depend on the term only (not on kappa),
+ * so they are computed once per term; each step only moves the descriptor.
+ */
+#define PT_QRUN_W 16  /* runs per wave */
 
 __device__ __forceinline__ void
-qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_RUNS_PER_WAVE],
-    int gb)
+qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W], int gb)
 {
 #pragma unroll
-	for (int n = 0; n < PT_RUNS_PER_WAVE; ++n) {
+	for (int n = 0; n < PT_QRUN_W; ++n) {
 		const int run = wave + 4 * n;
-		const int r = 16 * run + (lane >> 2);
-		const int u = r / 33, w = r - 33 * u;
+		const int u = run & 31, w = 16 * (run >> 5) + (lane >> 2);
 		const int ug = s.u0 + u, wg = s.w0 + w;
-		const bool v = run < PT_QRUNS && w < s.qwe && ug < s.uext &&
-		    wg < s.wext && (!s.packed || ug > wg);
-		const int lch = (lane & 3) ^ ((r >> 2) & 3);
+		const bool v = w < s.qwe && ug < s.uext && wg < s.wext &&
+		    (!s.packed || ug > wg);
 		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
 		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
-		vo[n] = v ? (uint32_t)(pidx * gb + lch * 16) : PT_OOR;
+		vo[n] = v ? (uint32_t)(pidx * gb + (lane & 3) * 16) : PT_OOR;
 	}
 }
 
 template <int GB, int KC>
 __device__ __forceinline__ void
 issue_q(const Step &s, int64_t qoff, int kc, char *sQ, int wave,
-    const uint32_t (&vo)[PT_RUNS_PER_WAVE])
+    const uint32_t (&vo)[PT_QRUN_W])
 {
 	const char *qb = (const char *)s.Q + qoff + (int64_t)(kc / KC) * s.q_k * GB;
 	const __amdgpu_buffer_rsrc_t rq = brsrc(qb, s.q_k * GB);
 #pragma unroll
-	for (int n = 0; n < PT_RUNS_PER_WAVE; ++n) {
+	for (int n = 0; n < PT_QRUN_W; ++n) {
 		const int run = wave + 4 * n;
-		if (run < PT_QRUNS)
+		const int h = run >> 5;
+		if (16 * h < s.qwe)
 			__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
-			    (__attribute__((address_space(3))) void *)(sQ + 1024 * run),
-			    16, vo[n], 0, 0, 0);
+			    (__attribute__((address_space(3))) void *)(sQ +
+			    (run & 31) * Stg::Q_USTR + 1024 * h), 16, vo[n], 0, 0, 0);
 	}
 }
 
@@ -436,7 +456,7 @@ wait_vm()
 	__asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-/* S rows of one step: load SEPT kappa per thread, store into a stage */
+/* S rows of one step: SEPT kappa per thread (sign applied), in registers */
 template <int F, int SEPT>
 __device__ __forceinline__ void
 load_s(const Step &s, int64_t soff, int kc, int tid,
@@ -463,17 +483,85 @@ template <int F, int SEPT>
 __device__ __forceinline__ void
 store_s(int tid, char *sS, const typename Form<F>::G (&sr)[SEPT])
 {
-	typedef Form<F> FM;
-	typedef Lds<F> LD;
-	typedef typename FM::L L;
-	constexpr int KC = FM::KC;
+	typedef typename Form<F>::G G;
+	constexpr int KC = Form<F>::KC;
 	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
 #pragma unroll
 	for (int e = 0; e < SEPT; ++e)
-		*(L *)(sS + LD::template eoff<(int)sizeof(L)>(row, k0 + e)) = sr[e];
+		*(G *)(sS + row * Stg::S_ROW + (k0 + e) * (int)sizeof(G)) = sr[e];
 }
 
-/* MFMA over one staged chunk into a fixed accumulator set */
+/* element e of a 16-byte chunk as an MFMA operand */
+template <class L>
+struct Chunk {
+	typedef L V __attribute__((ext_vector_type(16 / sizeof(L))));
+	static constexpr int N = 16 / sizeof(L);
+};
+
+/*
+ * Phase 1 (accumulator block x = c offset, D[a][b]) over one staged kappa
+ * chunk:  role a: A = S[a] (fixed), B = Q[b][c = x];
+ *         role b: A = Q[a][c = x],  B = S[b] (fixed).
+ */
+template <int F, int TCW>
+__device__ __forceinline__ void
+mma_p1(typename Form<F>::ACC (&ac)[TCW], bool ra, const char *sS,
+    const char *sQ, int wa, int wb, int ln, int lg)
+{
+	typedef typename Form<F>::L L;
+	typedef typename Chunk<L>::V V;
+	constexpr int NE = Chunk<L>::N;
+	const int ws = ra ? wa : wb, wu = ra ? wb : wa;
+	const V f = *(const V *)(sS + (16 * ws + ln) * Stg::S_ROW + 16 * lg);
+	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + 16 * lg;
+	if (ra) {
+#pragma unroll
+		for (int x = 0; x < TCW; ++x) {
+			const V q = *(const V *)(pv + 64 * x);
+#pragma unroll
+			for (int e = 0; e < NE; ++e)
+				ac[x] = mma(f[e], q[e], ac[x]);
+		}
+	} else {
+#pragma unroll
+		for (int x = 0; x < TCW; ++x) {
+			const V q = *(const V *)(pv + 64 * x);
+#pragma unroll
+			for (int e = 0; e < NE; ++e)
+				ac[x] = mma(q[e], f[e], ac[x]);
+		}
+	}
+}
+
+/*
+ * Phase 2 (block x = 16 cb + b, D[a][c]): role c: A = Q[a][b], B = S[c].
+ */
+template <int F, int TCW>
+__device__ __forceinline__ void
+mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
+    int wa, int wb, int ln, int lg)
+{
+	typedef typename Form<F>::L L;
+	typedef typename Chunk<L>::V V;
+	constexpr int NE = Chunk<L>::N, NCB = TCW / 16;
+	V f[NCB];
+#pragma unroll
+	for (int cb = 0; cb < NCB; ++cb)
+		f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 16 * lg);
+	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR + 16 * wb * 64 +
+	    16 * lg;
+#pragma unroll
+	for (int b = 0; b < 16; ++b) {
+		const V q = *(const V *)(pv + 64 * b);
+#pragma unroll
+		for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+			for (int e = 0; e < NE; ++e)
+				ac[16 * cb + b] = mma(q[e], f[cb][e], ac[16 * cb + b]);
+	}
+}
+
+/* split forms: MFMA over one staged chunk into a fixed accumulator set */
 template <int F, int NCB>
 __device__ __forceinline__ void
 mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
@@ -481,37 +569,8 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
 {
 	typedef Form<F> FM;
 	typedef Lds<F> LD;
-	typedef typename FM::L L;
-	constexpr int KC = FM::KC, KS = FM::KS, QP = LD::QP;
-	constexpr int TCW = 16 * NCB;
-	constexpr int ES = (int)sizeof(L);
-#define LDE(base, row, k) (*(const L *)((base) + LD::template eoff<ES>((row), (k))))
-	if constexpr (!FM::SPLIT) {
-		/* one branch-free MFMA sequence for all roles (a single flow of
-		 * the accumulators keeps them in place); block x reads
-		 *   A row ra0 + (x % 16) ra1 + (x / 16) ra2 of pa,
-		 *   B row rb0 + (x % 16) rb1 + (x / 16) rb2 of pb.
-		 * role a: A = S[a],    B = Q[b][c]
-		 * role b: A = Q[a][c], B = S[b]
-		 * role c: A = Q[a][b], B = S[c]  (block x = 16 cb + b) */
-		const bool ra = role == PT_ROLE_A, rb = role == PT_ROLE_B;
-		const char *pa = ra ? sS : sQ, *pb = ra ? sQ : sS;
-		const int ra0 = ra ? 16 * wa + ln :
-		    (16 * wa + ln) * QP + (rb ? 0 : 16 * wb);
-		const int ra1 = ra ? 0 : 1, ra2 = rb ? 16 : 0;
-		const int rb0 = ra ? (16 * wb + ln) * QP : rb ? 16 * wb + ln : ln;
-		const int rb1 = ra ? 1 : 0, rb2 = rb ? 0 : 16;
-#pragma unroll
-		for (int ks = 0; ks < KC; ks += KS) {
-			const int kap = ks + lg;
-#pragma unroll
-			for (int x = 0; x < TCW; ++x) {
-				const int rA = ra0 + (x & 15) * ra1 + (x >> 4) * ra2;
-				const int rB = rb0 + (x & 15) * rb1 + (x >> 4) * rb2;
-				ac[x] = mma(LDE(pa, rA, kap), LDE(pb, rB, kap), ac[x]);
-			}
-		}
-	} else {
+	constexpr int TCW = 16 * NCB, QP = LD::QP;
+	{
 #define LD8(base, plane, row) \
 	(*(const v8u *)((base) + (plane) + LD::off((row), lg)))
 		if (role == PT_ROLE_A) {
@@ -561,7 +620,6 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
 		}
 #undef LD8
 	}
-#undef LDE
 }
 
 /* -------------------------------------------------------------- kernel */
@@ -588,8 +646,11 @@ k_tiles(pt_launch p)
 	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
 	constexpr int NPC = 32 * 32 * (GB / 16) / PT_THREADS; /* Q pieces/thread */
 	constexpr int SEPT = 32 * KC / PT_THREADS; /* S elements per thread */
+	constexpr int SMEM = FM::SPLIT ? LD::BYTES : Stg::BYTES;
+	static_assert(SMEM >= 4 * NSET * TCW * 64 * (int)sizeof(AE),
+	    "epilogue slots exceed the stage area");
 
-	__shared__ __attribute__((aligned(16))) char smem[LD::BYTES];
+	__shared__ __attribute__((aligned(16))) char smem[SMEM];
 	__shared__ double red[4];
 	char *sS = smem;
 	char *sQ = smem + FM::NP * LD::S_PB;
@@ -659,57 +720,76 @@ k_tiles(pt_launch p)
 	};
 
 	if constexpr (!FM::SPLIT) {
-		/* double-buffered stages, Q streamed straight into LDS */
-		constexpr int STAGE = LD::STAGE_BYTES;
+		/* double-buffered stages; Q streamed straight into LDS, S through
+		 * registers (sign applied); the stage of step n + 1 loads while
+		 * step n computes */
+		constexpr int STAGE = Stg::STAGE;
 		const int wv = __builtin_amdgcn_readfirstlane(wave);
-		uint32_t vo[PT_RUNS_PER_WAVE];
+		uint32_t vo[PT_QRUN_W];
 		qtab(cur, wv, lane, vo, GB);
 		load_s<F, SEPT>(cur, soff, 0, tid, sr);
-		issue_q<GB, KC>(cur, qoff, 0, smem + LD::S_PB, wv, vo);
+		issue_q<GB, KC>(cur, qoff, 0, smem + Stg::S_BYTES, wv, vo);
 		wait_vm();
 		store_s<F, SEPT>(tid, smem, sr);
 		__syncthreads();
 		int buf = 0;
-		while (t < nterm) {
-			const int role = cur.role, set = cur.set;
-			int tn = t, kn = kc + KC;
-			if (kn >= cur.K) {
-				tn = t + 1;
-				kn = 0;
-			}
-			char *st_cur = smem + buf * STAGE;
-			char *st_nxt = smem + (buf ^ 1) * STAGE;
-			if (tn < nterm) {
-				if (tn != t) {
-					const pt_term &T = p.terms[tn];
-					cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
-					soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
-					qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
-					qtab(cur, wv, lane, vo, GB);
+		/* one loop per phase, so each holds a single MFMA pattern and the
+		 * accumulators stay in place; the transpose sits between them */
+		auto run_phase = [&](auto ph) {
+			constexpr int PH = decltype(ph)::value;
+			const int tend = PH == 1 ? p.nterm1 : nterm;
+			while (t < tend) {
+				const bool ra = cur.role == PT_ROLE_A;
+				const int set = cur.set;
+				int tn = t, kn = kc + KC;
+				if (kn >= cur.K) {
+					tn = t + 1;
+					kn = 0;
 				}
-				load_s<F, SEPT>(cur, soff, kn, tid, sr);
-				issue_q<GB, KC>(cur, qoff, kn, st_nxt + LD::S_PB, wv, vo);
+				const char *st_cur = smem + buf * STAGE;
+				char *st_nxt = smem + (buf ^ 1) * STAGE;
+				if (tn < nterm) {
+					if (tn != t) {
+						const pt_term &T = p.terms[tn];
+						cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
+						soff = T.s_off[0] * i + T.s_off[1] * j +
+						    T.s_off[2] * k;
+						qoff = T.q_off[0] * i + T.q_off[1] * j +
+						    T.q_off[2] * k;
+						qtab(cur, wv, lane, vo, GB);
+					}
+					load_s<F, SEPT>(cur, soff, kn, tid, sr);
+					issue_q<GB, KC>(cur, qoff, kn, st_nxt + Stg::S_BYTES, wv,
+					    vo);
+				}
+				const char *cS = st_cur, *cQ = st_cur + Stg::S_BYTES;
+				if constexpr (PH == 1) {
+					if (NSET == 1 || set == 0)
+						mma_p1<F, TCW>(acc[0], ra, cS, cQ, wa, wb, ln, lg);
+					else
+						mma_p1<F, TCW>(acc[NSET - 1], ra, cS, cQ, wa, wb,
+						    ln, lg);
+				} else {
+					if (NSET == 1 || set == 0)
+						mma_p2<F, TCW>(acc[0], cS, cQ, wa, wb, ln, lg);
+					else
+						mma_p2<F, TCW>(acc[NSET - 1], cS, cQ, wa, wb, ln,
+						    lg);
+				}
+				if (tn < nterm) {
+					wait_vm();
+					store_s<F, SEPT>(tid, st_nxt, sr);
+				}
+				__syncthreads();
+				t = tn;
+				kc = kn;
+				buf ^= 1;
 			}
-			if (act) {
-				if (NSET == 1 || set == 0)
-					mma_step<F, NCB>(acc[0], role, st_cur, st_cur + LD::S_PB,
-					    wa, wb, ln, lg);
-				else
-					mma_step<F, NCB>(acc[NSET - 1], role, st_cur,
-					    st_cur + LD::S_PB, wa, wb, ln, lg);
-			}
-			if (tn < nterm) {
-				wait_vm();
-				store_s<F, SEPT>(tid, st_nxt, sr);
-			}
-			__syncthreads();
-			/* phase boundary: the stage just consumed is scratch */
-			if (tn == p.nterm1 && kn == 0 && tn < nterm)
-				transpose((AE *)st_cur);
-			t = tn;
-			kc = kn;
-			buf ^= 1;
-		}
+		};
+		run_phase(std::integral_constant<int, 1>());
+		/* the stage consumed last is free: transpose through it */
+		transpose((AE *)(smem + (buf ^ 1) * STAGE));
+		run_phase(std::integral_constant<int, 2>());
 	} else {
 		v4u qr[NPC];
 		load_step<F, NPC, SEPT>(cur, soff, qoff, 0, tid, qr, sr);
@@ -968,27 +1048,44 @@ k_synth(pt_synth_spec spec, int tensor, void *dst, int64_t n, int fp32)
 	}
 }
 
+template <int F, int NSET>
+int
+launch_kind(const pt_launch *p, hipStream_t s)
+{
+	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
+	if (p->kind == PT_AAA)
+		k_tiles<F, PT_AAA, NSET><<<grid, block, 0, s>>>(*p);
+	else
+		k_tiles<F, PT_AAB, NSET><<<grid, block, 0, s>>>(*p);
+	return (int)hipGetLastError();
+}
+
+#ifndef PT_KERNEL_F64X2
 template <int F>
 int
 launch_form(const pt_launch *p, hipStream_t s)
 {
-	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
-	if (p->kind == PT_AAA) {
-		if (p->nset == 1)
-			k_tiles<F, PT_AAA, 1><<<grid, block, 0, s>>>(*p);
-		else
-			k_tiles<F, PT_AAA, 2><<<grid, block, 0, s>>>(*p);
-	} else {
-		if (p->nset == 1)
-			k_tiles<F, PT_AAB, 1><<<grid, block, 0, s>>>(*p);
-		else
-			k_tiles<F, PT_AAB, 2><<<grid, block, 0, s>>>(*p);
-	}
-	return (int)hipGetLastError();
+	if (p->nset == 1)
+		return launch_kind<F, 1>(p, s);
+	/* FP64 with two W sets is built in a translation unit of its own,
+	 * without the VGPR-form MFMA option (its accumulators alone fill 256
+	 * registers, and that option gives wrong results there) */
+	if constexpr (F == PT_FORM_FP64)
+		return pt_launch_tiles_f64x2(p, s);
+	else
+		return launch_kind<F, 2>(p, s);
 }
+#endif
 
 } /* namespace */
 
+#ifdef PT_KERNEL_F64X2
+extern "C" int
+pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream)
+{
+	return launch_kind<PT_FORM_FP64, 2>(p, (hipStream_t)stream);
+}
+#else
 extern "C" int
 pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 {
@@ -1079,3 +1176,4 @@ pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n, int fp32,
 	    fp32);
 	return (int)hipGetLastError();
 }
+#endif /* PT_KERNEL_F64X2 */
