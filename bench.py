@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -99,6 +101,37 @@ def run_form(api, form, dims, dev, args, nbatch):
     return res
 
 
+def run_batch(api, form, n, seed):
+    """n independent restricted systems (o in [4, 20], v in [40, 160]) in
+    one graph launch; one untimed replay first."""
+    dims = api.batch_dims(n, seed=seed)
+    systems = [(d, api.synth_device("rpt", api.make_spec("rpt", d,
+                                                         seed=seed + 1 + k),
+                                    fp32=form == "fp32"))
+               for k, d in enumerate(dims)]
+    try:
+        t0 = time.perf_counter()
+        b = api.Batch("rpt", form, systems)
+        prep = time.perf_counter() - t0
+        with b:
+            b.run()
+            b.energies()
+            t0 = time.perf_counter()
+            b.run()
+            e = b.energies()
+            wall = time.perf_counter() - t0
+            ms = b.last_ms()
+            flop, items = b.flop, b.items
+    finally:
+        for _, dev in systems:
+            dev.close()
+    return {"systems": n, "o_range": [4, 20], "v_range": [40, 160],
+            "form": form, "items": items, "flop": flop,
+            "prep_s": round(prep, 3), "graph_ms": round(ms, 3),
+            "wall_s": round(wall, 4), "tflops": round(flop / ms / 1e9, 3),
+            "E_sum": float(np.sum(e))}
+
+
 def cpu_baseline(api, o, v, seed):
     """The reference libpt compiled from its sources (OpenMP, MKL in
     sequential mode) on a bounded sample; falls back to the oracle."""
@@ -132,6 +165,8 @@ def main():
     ap.add_argument("--nbatch", type=int, default=128)
     ap.add_argument("--forms", default="fp64,bf16x3")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--batch", type=int, default=64,
+                    help="systems in the batched-launch measurement (0: skip)")
     ap.add_argument("--cpu", default="16,160",
                     help="o,v of the CPU-baseline sample; 'none' to skip")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -225,6 +260,11 @@ def main():
             "rel_dE": abs(q["e_batches"] - r["e_batches"]) /
                       abs(r["e_batches"]),
         }
+    if args.batch > 0:
+        out["batch"] = {}
+        for form in forms:
+            out["batch"][form] = run_batch(api, form, args.batch, args.seed)
+            log(f"[bench] batch {form}: {out['batch'][form]}")
     if args.cpu != "none":
         co, cv = (int(x) for x in args.cpu.split(","))
         out["cpu_baseline"] = cpu_baseline(api, co, cv, args.seed)

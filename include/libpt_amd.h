@@ -84,6 +84,30 @@ int libpt_amd_plan_ijk_list(const libpt_amd_plan *plan, int sc, int32_t *out);
 int libpt_amd_plan_timer_begin(libpt_amd_plan *plan);
 double libpt_amd_plan_timer_end_ms(libpt_amd_plan *plan);
 
+/* ---- batches of independent systems ---- */
+typedef struct libpt_amd_batch libpt_amd_batch;
+
+/* n independent systems of one call kind, in one arithmetic form.
+ * dims[4 s .. 4 s + 3] = (oa, va, ob, vb) of system s (restricted calls
+ * read oa, va); dev_arrays[NA s + t] = its t-th pt.h array in device memory
+ * (NA = 7 for rpt/upt, 10 for rft/uft).  Stages every system's operands
+ * and records one graph: a single tile launch over every (system, spin
+ * case, ijk, tile) item, largest items first, then one reduction per spin
+ * case.  NULL on error. */
+libpt_amd_batch *libpt_amd_batch_create(int call, int form, int n,
+    const int64_t *dims, const void *const *dev_arrays);
+void libpt_amd_batch_destroy(libpt_amd_batch *batch);
+/* enqueue one launch of the batch's graph (asynchronous); 0 on success */
+int libpt_amd_batch_run(libpt_amd_batch *batch);
+/* energies of the n systems from the last run (synchronises); each equals
+ * libpt_amd_plan_energy of that system alone, bit for bit */
+int libpt_amd_batch_energies(libpt_amd_batch *batch, double *energies);
+/* GPU time of the last run in ms (synchronises) */
+double libpt_amd_batch_last_ms(libpt_amd_batch *batch);
+/* reference-formulation flops of one run, and its work items */
+double libpt_amd_batch_flop(const libpt_amd_batch *batch);
+int64_t libpt_amd_batch_items(const libpt_amd_batch *batch);
+
 /* ---- device memory and synthetic inputs ---- */
 void *libpt_amd_dev_alloc(size_t bytes);
 void libpt_amd_dev_free(void *ptr);

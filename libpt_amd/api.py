@@ -99,6 +99,20 @@ def lib():
         L.libpt_amd_plan_timer_begin.argtypes = [_p]
         L.libpt_amd_plan_timer_end_ms.restype = _d
         L.libpt_amd_plan_timer_end_ms.argtypes = [_p]
+        L.libpt_amd_batch_create.restype = _p
+        L.libpt_amd_batch_create.argtypes = [_i, _i, _i, C.POINTER(_i64),
+                                             C.POINTER(_p)]
+        L.libpt_amd_batch_destroy.argtypes = [_p]
+        L.libpt_amd_batch_run.restype = _i
+        L.libpt_amd_batch_run.argtypes = [_p]
+        L.libpt_amd_batch_energies.restype = _i
+        L.libpt_amd_batch_energies.argtypes = [_p, _p]
+        L.libpt_amd_batch_last_ms.restype = _d
+        L.libpt_amd_batch_last_ms.argtypes = [_p]
+        L.libpt_amd_batch_flop.restype = _d
+        L.libpt_amd_batch_flop.argtypes = [_p]
+        L.libpt_amd_batch_items.restype = _i64
+        L.libpt_amd_batch_items.argtypes = [_p]
         L.libpt_amd_dev_alloc.restype = _p
         L.libpt_amd_dev_alloc.argtypes = [_sz]
         L.libpt_amd_dev_free.argtypes = [_p]
@@ -348,6 +362,66 @@ class Plan:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Batch:
+    """Many independent systems of one call kind in one graph launch
+    (libpt_amd_batch_*).  `systems` is a list of (dims, DeviceArrays)."""
+
+    def __init__(self, kind, form, systems):
+        form = FORMS.get(form, form)
+        self.kind, self.form, self.n = kind, form, len(systems)
+        dims = []
+        ptrs = []
+        for d, dev in systems:
+            dims.extend(_dims4(kind, d))
+            ptrs.extend(dev.ptrs)
+        cd = (_i64 * len(dims))(*dims)
+        cp = (_p * len(ptrs))(*ptrs)
+        self.h = lib().libpt_amd_batch_create(CALLS[kind], form, self.n, cd,
+                                              cp)
+        if not self.h:
+            raise LibptError(last_error())
+
+    def run(self):
+        if lib().libpt_amd_batch_run(self.h) != 0:
+            raise LibptError(last_error())
+
+    def energies(self):
+        out = np.zeros(self.n)
+        if lib().libpt_amd_batch_energies(self.h, out.ctypes.data) != 0:
+            raise LibptError(last_error())
+        return out
+
+    def last_ms(self):
+        return lib().libpt_amd_batch_last_ms(self.h)
+
+    @property
+    def flop(self):
+        return lib().libpt_amd_batch_flop(self.h)
+
+    @property
+    def items(self):
+        return lib().libpt_amd_batch_items(self.h)
+
+    def close(self):
+        if self.h:
+            lib().libpt_amd_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def batch_dims(n=64, seed=64, o=(4, 20), v=(40, 160)):
+    """Sizes of a batch of restricted systems: o and v uniform integers in
+    the given closed ranges, from a seeded generator."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(o[0], o[1] + 1)), int(rng.integers(v[0], v[1] + 1)))
+            for _ in range(n)]
 
 
 def device_copy_calibration(n_doubles: int) -> None:

@@ -113,6 +113,16 @@ extern "C" {
 int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
 /* FP64 tile kernels with two W sets (built in a separate unit) */
 int pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream);
+/* batched launch: device array of nseg launches (partial, ijk, tiles,
+ * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
+ * every item of every segment in one grid (several above 2^22 items) */
+int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
+    const int64_t *begin, int nseg, int64_t total, void *stream);
+int pt_launch_batch_f64x2(const struct pt_launch *segs,
+    const int64_t *begin, int nseg, int64_t total, void *stream);
+/* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
+int pt_launch_reduce_batch(const struct pt_launch *segs,
+    const int64_t *begin, int nseg, double *out, void *stream);
 int pt_launch_reduce(const double *x, int64_t n, double *out, void *stream);
 int pt_launch_segsum(const double *x, int64_t nseg, int64_t seglen,
     double *out, void *stream);

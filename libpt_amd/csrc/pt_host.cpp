@@ -18,6 +18,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -759,6 +760,235 @@ libpt_amd_plan_timer_end_ms(libpt_amd_plan *pl)
 	HCHK(hipEventSynchronize(pl->ev1), NAN);
 	HCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1), NAN);
 	return (double)ms;
+}
+
+/* ---------------------------------------------------------------------
+ * Batches: many independent systems in one graph launch.  Each system is a
+ * plan (operands staged and packed once, at creation); a segment is one
+ * spin case of one system.  The graph holds one batched tile launch over
+ * every item of every segment, ordered by cost per item (largest first, so
+ * the small items of small systems fill the tail), and one batched
+ * reduction that leaves one fixed-order sum per segment.
+ */
+struct libpt_amd_batch {
+	int call = 0, form = 0, n = 0, nset = 1;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev0 = nullptr, ev1 = nullptr;
+	std::vector<libpt_amd_plan *> plans;
+	std::vector<int> seg_sys;
+	std::vector<int> seg_of;      /* [system][spin case] -> segment or -1 */
+	std::vector<double> seg_factor;
+	pt_launch *d_segs = nullptr;
+	int64_t *d_begin = nullptr;
+	double *d_out = nullptr;
+	int64_t total = 0;
+	double flop = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void
+batch_free(libpt_amd_batch *b)
+{
+	if (!b)
+		return;
+	if (b->stream)
+		(void)hipStreamSynchronize(b->stream);
+	if (b->exec)
+		(void)hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void)hipGraphDestroy(b->graph);
+	for (libpt_amd_plan *pl : b->plans)
+		plan_free(pl);
+	(void)hipFree(b->d_segs);
+	(void)hipFree(b->d_begin);
+	(void)hipFree(b->d_out);
+	if (b->ev0)
+		(void)hipEventDestroy(b->ev0);
+	if (b->ev1)
+		(void)hipEventDestroy(b->ev1);
+	if (b->stream)
+		(void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+} /* namespace */
+
+extern "C" libpt_amd_batch *
+libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
+    const void *const *dev_arrays)
+{
+	if (!dev_init())
+		return nullptr;
+	if (n <= 0 || !dims || !dev_arrays) {
+		set_err("empty batch");
+		return nullptr;
+	}
+	const bool ft = call == LIBPT_AMD_RFT || call == LIBPT_AMD_UFT;
+	const int na = ft ? 10 : 7;
+	libpt_amd_batch *b = new libpt_amd_batch;
+	b->call = call;
+	b->form = form;
+	b->n = n;
+	b->nset = ft ? 2 : 1;
+#define BFAIL(cond)                   \
+	do {                          \
+		if (!(cond)) {        \
+			batch_free(b); \
+			return nullptr; \
+		}                     \
+	} while (0)
+	BFAIL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) ==
+	    hipSuccess || (set_err("hipStreamCreate failed"), false));
+	BFAIL(hipEventCreate(&b->ev0) == hipSuccess &&
+	    hipEventCreate(&b->ev1) == hipSuccess);
+
+	struct Seg {
+		pt_launch L;
+		int sys, sc;
+		double factor, cost;
+		int64_t items;
+	};
+	std::vector<Seg> segs;
+	for (int s = 0; s < n; ++s) {
+		const int64_t *d = dims + 4 * s;
+		libpt_amd_plan *pl = libpt_amd_plan_create(call, form, (size_t)d[0],
+		    (size_t)d[1], (size_t)d[2], (size_t)d[3], dev_arrays + na * s);
+		BFAIL(pl != nullptr);
+		b->plans.push_back(pl);
+		for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
+			Case &c = pl->cases[sc];
+			b->flop += c.flop_per_ijk * (double)c.n_ijk;
+			if (c.n_ijk == 0 || c.n_tiles == 0)
+				continue;
+			Seg g;
+			g.sc = sc;
+			g.L = c.L;
+			g.L.item0 = 0;
+			g.items = c.n_ijk * c.n_tiles;
+			g.L.n_chunk = g.items;
+			g.sys = s;
+			g.factor = c.factor;
+			g.cost = c.flop_per_ijk / c.n_tiles;
+			segs.push_back(g);
+		}
+	}
+	/* largest items first; ties keep system order */
+	std::stable_sort(segs.begin(), segs.end(),
+	    [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
+	const int nseg = (int)segs.size();
+	std::vector<pt_launch> L(nseg);
+	std::vector<int64_t> begin(nseg + 1, 0);
+	for (int g = 0; g < nseg; ++g) {
+		L[g] = segs[g].L;
+		begin[g + 1] = begin[g] + segs[g].items;
+		b->seg_sys.push_back(segs[g].sys);
+		b->seg_factor.push_back(segs[g].factor);
+	}
+	b->seg_of.assign((size_t)n * 4, -1);
+	for (int g = 0; g < nseg; ++g)
+		b->seg_of[(size_t)segs[g].sys * 4 + segs[g].sc] = g;
+	b->total = begin[nseg];
+	BFAIL(dev_alloc((void **)&b->d_segs, sizeof(pt_launch) * (nseg + 1)));
+	BFAIL(dev_alloc((void **)&b->d_begin, sizeof(int64_t) * (nseg + 1)));
+	BFAIL(dev_alloc((void **)&b->d_out, sizeof(double) * (nseg + 1)));
+	if (nseg > 0) {
+		BFAIL(hipMemcpy(b->d_segs, L.data(), sizeof(pt_launch) * nseg,
+		    hipMemcpyHostToDevice) == hipSuccess ||
+		    (set_err("segment upload failed"), false));
+		BFAIL(hipMemcpy(b->d_begin, begin.data(),
+		    sizeof(int64_t) * (nseg + 1), hipMemcpyHostToDevice) ==
+		    hipSuccess || (set_err("segment upload failed"), false));
+	}
+
+	/* the graph: batched tiles, then batched reduction */
+	BFAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) ==
+	    hipSuccess || (set_err("stream capture failed"), false));
+	int rc = pt_launch_batch(form, b->nset, b->d_segs, b->d_begin, nseg,
+	    b->total, b->stream);
+	if (rc == 0)
+		rc = pt_launch_reduce_batch(b->d_segs, b->d_begin, nseg, b->d_out,
+		    b->stream);
+	hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+	BFAIL((rc == 0 && ec == hipSuccess) ||
+	    (set_err("batch capture failed: %s",
+	    hipGetErrorString(rc ? (hipError_t)rc : ec)), false));
+	BFAIL(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) ==
+	    hipSuccess || (set_err("graph instantiation failed"), false));
+#undef BFAIL
+	return b;
+}
+
+extern "C" void
+libpt_amd_batch_destroy(libpt_amd_batch *b)
+{
+	batch_free(b);
+}
+
+extern "C" double
+libpt_amd_batch_flop(const libpt_amd_batch *b)
+{
+	return b ? b->flop : 0.0;
+}
+
+extern "C" int64_t
+libpt_amd_batch_items(const libpt_amd_batch *b)
+{
+	return b ? b->total : 0;
+}
+
+/* GPU time of the last libpt_amd_batch_run, ms (synchronises) */
+extern "C" double
+libpt_amd_batch_last_ms(libpt_amd_batch *b)
+{
+	if (!b)
+		return NAN;
+	float ms = 0;
+	HCHK(hipEventSynchronize(b->ev1), NAN);
+	HCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1), NAN);
+	return (double)ms;
+}
+
+extern "C" int
+libpt_amd_batch_run(libpt_amd_batch *b)
+{
+	if (!b) {
+		set_err("null batch");
+		return -1;
+	}
+	HCHK(hipEventRecord(b->ev0, b->stream), -1);
+	HCHK(hipGraphLaunch(b->exec, b->stream), -1);
+	HCHK(hipEventRecord(b->ev1, b->stream), -1);
+	return 0;
+}
+
+extern "C" int
+libpt_amd_batch_energies(libpt_amd_batch *b, double *out)
+{
+	if (!b || !out) {
+		set_err("null batch or output");
+		return -1;
+	}
+	const int nseg = (int)b->seg_sys.size();
+	std::vector<double> e(nseg > 0 ? nseg : 1);
+	if (nseg > 0)
+		HCHK(hipMemcpyAsync(e.data(), b->d_out, sizeof(double) * nseg,
+		    hipMemcpyDeviceToHost, b->stream), -1);
+	HCHK(hipStreamSynchronize(b->stream), -1);
+	/* spin cases of one system in plan order, as libpt_amd_plan_energy */
+	for (int s = 0; s < b->n; ++s) {
+		double x = 0.0;
+		const int ncase = (int)b->plans[s]->cases.size();
+		for (int ci = 0; ci < ncase; ++ci) {
+			const int g = b->seg_of[(size_t)s * 4 + ci];
+			if (g >= 0)
+				x += b->seg_factor[g] * e[g];
+		}
+		out[s] = x;
+	}
+	return 0;
 }
 
 extern "C" void *

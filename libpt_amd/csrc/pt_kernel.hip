@@ -629,9 +629,15 @@ template <int F, int NSET> struct Occ {
 	static constexpr int WG_PER_CU = 1;
 };
 
+/*
+ * One work item (ijk, tile) of a spin case: everything between reading the
+ * item and writing its energy to partial[it].  KIND is PT_AAA, PT_AAB, or
+ * -1 to take the spin case from p.kind at run time (batched launches that
+ * mix both cases; only the epilogue depends on it).
+ */
 template <int F, int KIND, int NSET>
-__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
-k_tiles(pt_launch p)
+__device__ __forceinline__ void
+tile_item(const pt_launch &p, int64_t it)
 {
 	typedef Form<F> FM;
 	typedef Lds<F> LD;
@@ -660,13 +666,7 @@ k_tiles(pt_launch p)
 	const int wa = wave & 1, wb = wave >> 1;
 	const int ln = lane & 15, lg = lane >> 4;
 
-	/* XCD-aware item order: workgroup n runs on XCD n % 8; each XCD gets a
-	 * contiguous run of items, so neighbouring tiles of one ijk share L2 */
-	const int64_t per = (p.n_chunk + 7) / 8;
-	int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-	if (it >= p.n_chunk)
-		return;
-	it += p.item0;
+	const bool same = (KIND >= 0 ? KIND : p.kind) == PT_AAA;
 	const int64_t ijk_id = it / p.n_tiles;
 	const int tile = (int)(it - ijk_id * p.n_tiles);
 	const int i = p.ijk[3 * ijk_id], j = p.ijk[3 * ijk_id + 1];
@@ -677,7 +677,7 @@ k_tiles(pt_launch p)
 	const int a_lo = A0 + 16 * wa, b_lo = B0 + 16 * wb;
 	/* does this wave's block hold any element of the triangle? */
 	const bool act = a_lo < na && b_lo < na && a_lo + 15 > b_lo &&
-	    (KIND != PT_AAA || b_lo + 15 > C0);
+	    (!same || b_lo + 15 > C0);
 
 	ACC acc[NSET][TCW];
 #pragma unroll
@@ -849,11 +849,11 @@ k_tiles(pt_launch p)
 		for (int x = 0; x < TCW; ++x) {
 			const int b = b_lo + (x & 15);
 			const int c = C0 + 16 * (x >> 4) + ln;
-			if (!(c < nc && a > b && (KIND != PT_AAA || b > c)))
+			if (!(c < nc && a > b && (!same || b > c)))
 				continue;
 			ER z, dn;
 			const ER *d1 = (const ER *)p.epi.d1;
-			if (KIND == PT_AAA) {
+			if (same) {
 				const int64_t o = p.oa, v = p.va;
 				z = z_same<ER>((const ER *)p.epi.u1, (const ER *)p.epi.GA,
 				    o, v, i, j, k, a, b, c) +
@@ -896,9 +896,49 @@ k_tiles(pt_launch p)
 		p.partial[it] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-/* one workgroup, fixed-order compensated sum of x[0..n) */
-__global__ __launch_bounds__(1024) void
-k_reduce(const double *x, int64_t n, double *out)
+/* one launch over items [item0, item0 + n_chunk) of one spin case.  XCD-
+ * aware order: workgroup n runs on XCD n % 8, and each XCD gets a
+ * contiguous run of items, so neighbouring tiles of one ijk share L2. */
+template <int F, int KIND, int NSET>
+__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
+k_tiles(pt_launch p)
+{
+	const int64_t per = (p.n_chunk + 7) / 8;
+	const int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+	if (it >= p.n_chunk)
+		return;
+	tile_item<F, KIND, NSET>(p, p.item0 + it);
+}
+
+/*
+ * One launch over the items of many segments (a segment = one spin case of
+ * one system, both kinds mixed).  Segment s owns global items
+ * [begin[s], begin[s + 1]); workgroup g takes global item g0 + g, so the
+ * hardware dispatcher hands items out in segment order (the host sorts
+ * segments by cost per item, largest first).
+ */
+template <int F, int NSET>
+__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
+k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
+    int64_t g0, int64_t n)
+{
+	if ((int64_t)blockIdx.x >= n)
+		return;
+	const int64_t g = g0 + blockIdx.x;
+	int lo = 0, hi = nseg - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (begin[mid] <= g)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	tile_item<F, -1, NSET>(segs[lo], g - begin[lo]);
+}
+
+/* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
+__device__ __forceinline__ void
+reduce_block(const double *x, int64_t n, double *out)
 {
 	__shared__ double sh[1024];
 	double s = 0.0, c = 0.0;
@@ -913,6 +953,21 @@ k_reduce(const double *x, int64_t n, double *out)
 	}
 	if (threadIdx.x == 0)
 		out[0] = sh[0];
+}
+
+/* one workgroup, fixed-order compensated sum of x[0..n) */
+__global__ __launch_bounds__(1024) void
+k_reduce(const double *x, int64_t n, double *out)
+{
+	reduce_block(x, n, out);
+}
+
+/* one workgroup per segment: out[s] = the k_reduce sum of its partials */
+__global__ __launch_bounds__(1024) void
+k_reduce_batch(const pt_launch *segs, const int64_t *begin, double *out)
+{
+	const int s = blockIdx.x;
+	reduce_block(segs[s].partial, begin[s + 1] - begin[s], out + s);
 }
 
 /* out[s] = sum of segment s (in order) */
@@ -1060,6 +1115,21 @@ launch_kind(const pt_launch *p, hipStream_t s)
 	return (int)hipGetLastError();
 }
 
+template <int F, int NSET>
+int
+launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
+    int64_t total, hipStream_t s)
+{
+	/* grids of at most 2^22 workgroups */
+	const int64_t chunk = (int64_t)1 << 22;
+	for (int64_t g0 = 0; g0 < total; g0 += chunk) {
+		const int64_t n = total - g0 < chunk ? total - g0 : chunk;
+		k_tiles_batch<F, NSET><<<dim3((unsigned)n), dim3(PT_THREADS), 0,
+		    s>>>(segs, begin, nseg, g0, n);
+	}
+	return (int)hipGetLastError();
+}
+
 #ifndef PT_KERNEL_F64X2
 template <int F>
 int
@@ -1085,6 +1155,14 @@ pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream)
 {
 	return launch_kind<PT_FORM_FP64, 2>(p, (hipStream_t)stream);
 }
+
+extern "C" int
+pt_launch_batch_f64x2(const struct pt_launch *segs, const int64_t *begin,
+    int nseg, int64_t total, void *stream)
+{
+	return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
+	    (hipStream_t)stream);
+}
 #else
 extern "C" int
 pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
@@ -1099,6 +1177,37 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
 	}
 	return (int)hipErrorInvalidValue;
+}
+
+extern "C" int
+pt_launch_batch(int form, int nset, const struct pt_launch *segs,
+    const int64_t *begin, int nseg, int64_t total, void *stream)
+{
+	hipStream_t s = (hipStream_t)stream;
+	if (nseg <= 0 || total <= 0)
+		return 0;
+	if (nset == 2 && form == PT_FORM_FP64)
+		return pt_launch_batch_f64x2(segs, begin, nseg, total, stream);
+	switch (form * 2 + (nset - 1)) {
+	case 0: return launch_batch<PT_FORM_FP64, 1>(segs, begin, nseg, total, s);
+	case 2: return launch_batch<PT_FORM_FP32, 1>(segs, begin, nseg, total, s);
+	case 3: return launch_batch<PT_FORM_FP32, 2>(segs, begin, nseg, total, s);
+	case 4: return launch_batch<PT_FORM_BF16X3, 1>(segs, begin, nseg, total, s);
+	case 5: return launch_batch<PT_FORM_BF16X3, 2>(segs, begin, nseg, total, s);
+	case 6: return launch_batch<PT_FORM_F16X3, 1>(segs, begin, nseg, total, s);
+	case 7: return launch_batch<PT_FORM_F16X3, 2>(segs, begin, nseg, total, s);
+	}
+	return (int)hipErrorInvalidValue;
+}
+
+extern "C" int
+pt_launch_reduce_batch(const struct pt_launch *segs, const int64_t *begin,
+    int nseg, double *out, void *stream)
+{
+	if (nseg <= 0)
+		return 0;
+	k_reduce_batch<<<nseg, 1024, 0, (hipStream_t)stream>>>(segs, begin, out);
+	return (int)hipGetLastError();
 }
 
 extern "C" int
