@@ -9,8 +9,9 @@ host-only sources go through the same driver so they see the HIP headers.
 The kernel source is also built with ``-amdgpu-mfma-vgpr-form``: MFMA
 accumulators then live in ordinary VGPRs, which keeps the tile kernels'
 loop-carried accumulators from being copied into AGPRs on every step.  The
-FP64 two-W-set kernels (256 accumulator registers) come from
-``pt_kernel_f64x2.hip``, the same source built without that option.
+FP64 two-W-set kernels (256 accumulator registers) and the split-form
+kernels come from ``pt_kernel_agpr.hip``, the same source built without
+that option.
 """
 
 from __future__ import annotations
@@ -25,7 +26,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libpt_amd.so")
 
-SOURCES = ["pt_kernel.hip", "pt_kernel_f64x2.hip", "pt_host.cpp", "pt_terms.cpp"]
+SOURCES = ["pt_kernel.hip", "pt_kernel_agpr.hip", "pt_host.cpp", "pt_terms.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

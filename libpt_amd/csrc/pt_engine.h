@@ -91,12 +91,14 @@ struct pt_epi {
 
 struct pt_launch {
 	const void *buf[PT_MAX_BUFS];
-	const struct pt_term *terms; /* device array: phase-1 terms, then phase 2 */
+	const struct pt_term *terms; /* device array: role a, role b, role c */
 	int32_t nterm1, nterm2;
 	int32_t kind, nset;
 	int64_t oa, va, ob, vb;      /* aaa: ob = oa, vb = va */
 	int32_t na, nc;              /* extents of a (= b) and c */
-	int32_t tcw, pad0_;          /* c tile width (PT_TCW_T or PT_TCW_FT) */
+	int32_t tcw;                 /* c tile width (pt_tcw) */
+	int32_t nterm_a;             /* phase-1 terms with role a (they come
+	                                first, then role b) */
 	const int32_t *ijk;          /* [n_ijk][3] */
 	const int32_t *tiles;        /* [n_tiles][3]: blocks (A, B, C), A >= B */
 	int32_t n_tiles, pad_;
@@ -111,14 +113,15 @@ extern "C" {
 #endif
 /* kernel launchers (pt_kernel.hip); return hipError_t as int */
 int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
-/* FP64 tile kernels with two W sets (built in a separate unit) */
-int pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream);
+/* the kernels built without VGPR-form MFMA (pt_kernel_agpr.hip): FP64
+ * with two W sets and the split forms */
+int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
 /* batched launch: device array of nseg launches (partial, ijk, tiles,
  * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
  * every item of every segment in one grid (several above 2^22 items) */
 int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
     const int64_t *begin, int nseg, int64_t total, void *stream);
-int pt_launch_batch_f64x2(const struct pt_launch *segs,
+int pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
     const int64_t *begin, int nseg, int64_t total, void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,

@@ -147,7 +147,29 @@ flop_mixed(double oa, double va, double ob, double vb)
 	    4 * ob * va * va * vb;
 }
 
+/* blocking of the tile order (A, B, C tiles per block) */
+struct TileBlock {
+	int ga = 2, gb = 2, gc = 8;
+};
+
+/* LIBPT_AMD_TILE_BLOCK="ga,gb,gc" overrides the default blocking */
+TileBlock
+tile_block_default()
+{
+	TileBlock tb;
+	const char *e = getenv("LIBPT_AMD_TILE_BLOCK");
+	int a, b, c;
+	if (e && sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && a > 0 && b > 0 &&
+	    c > 0) {
+		tb.ga = a;
+		tb.gb = b;
+		tb.gc = c;
+	}
+	return tb;
+}
+
 struct Case {
+	TileBlock block = tile_block_default();
 	pt_launch L;
 	std::vector<pt_term> terms;
 	pt_term *d_terms = nullptr;
@@ -241,18 +263,26 @@ case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na, int64_t nc,
 	HostBuf<int32_t> tiles;
 	if (!tiles.alloc(3 * (size_t)nA * (nA + 1) / 2 * nC))
 		return false;
+	/* Tiles in blocks of GA x GB x GC (A, B, C): the workgroups that run
+	 * at the same time on one XCD take consecutive items, and a compact
+	 * block lets them share the Q slices of all three roles in L2
+	 * (Q[B][C] across A, Q[A][C] across B, Q[A][B] across C). */
+	const TileBlock tb = c.block;
 	int nt = 0;
-	for (int A = 0; A < nA; ++A)
-		for (int B = 0; B <= A; ++B)
-			for (int Cb = 0; Cb < nC; ++Cb) {
-				/* same spin needs some b > c: c_min < b_max */
-				if (same && Cb * tcw >= B * PT_TB + PT_TB - 1)
-					continue;
-				tiles.p[3 * nt] = A;
-				tiles.p[3 * nt + 1] = B;
-				tiles.p[3 * nt + 2] = Cb;
-				++nt;
-			}
+	for (int A0 = 0; A0 < nA; A0 += tb.ga)
+	for (int B0 = 0; B0 < A0 + tb.ga && B0 < nA; B0 += tb.gb)
+	for (int C0 = 0; C0 < nC; C0 += tb.gc)
+		for (int A = A0; A < A0 + tb.ga && A < nA; ++A)
+		for (int B = B0; B < B0 + tb.gb && B <= A; ++B)
+		for (int Cb = C0; Cb < C0 + tb.gc && Cb < nC; ++Cb) {
+			/* same spin needs some b > c: c_min < b_max */
+			if (same && Cb * tcw >= B * PT_TB + PT_TB - 1)
+				continue;
+			tiles.p[3 * nt] = A;
+			tiles.p[3 * nt + 1] = B;
+			tiles.p[3 * nt + 2] = Cb;
+			++nt;
+		}
 	c.n_ijk = nijk;
 	c.n_tiles = nt;
 	if (!dev_alloc((void **)&c.d_ijk, 3 * sizeof(int32_t) * (size_t)nijk) ||
@@ -298,7 +328,7 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 		for (int x = 0; x < 3; ++x)
 			t.q_off[x] = t.q_off[x] / slab * pslab;
 	}
-	L.nterm1 = pt_terms_order(c.terms);
+	L.nterm1 = pt_terms_order(c.terms, &L.nterm_a);
 	L.nterm2 = (int32_t)c.terms.size() - L.nterm1;
 	if (!dev_alloc((void **)&c.d_terms, sizeof(pt_term) * c.terms.size()))
 		return false;

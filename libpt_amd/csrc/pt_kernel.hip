@@ -37,6 +37,10 @@
 #include "pt_engine.h"
 #include "synth.h"
 
+#ifndef PT_KERNEL_AGPR
+#define PT_KERNEL_AGPR 0
+#endif
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -205,37 +209,59 @@ kahan(double &s, double &cmp, double x)
 
 /* ------------------------------------------------- one staging step */
 
-/* Operand geometry of one term inside the workgroup tile (uniform). */
+/* values the whole wave shares: readfirstlane tells the compiler so, which
+ * keeps them (and everything derived from them) in scalar registers */
+__device__ __forceinline__ int
+uni(int x)
+{
+	return __builtin_amdgcn_readfirstlane(x);
+}
+
+__device__ __forceinline__ int64_t
+uni64(int64_t x)
+{
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+	return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+/* Operand geometry of one term inside the workgroup tile (uniform), with
+ * the byte bases of its S rows and of kappa group 0 of its packed Q slab;
+ * a step (kappa chunk kc) then only moves both bases. */
 struct Step {
-	const void *S, *Q;
-	int64_t s_st, q_k, q_u, q_w;
+	const char *sb0;     /* S element (s = 0, kappa = 0) of this ijk */
+	const char *qb0;     /* packed Q group 0 of this ijk */
+	int64_t sbytes;      /* S rows [0, sext) from sb0 */
+	int64_t qgs;         /* bytes per kappa group of the packed slab */
+	int64_t s_st, q_u, q_w;
 	int role, set, K, neg, packed, ufast;
 	int s0, sn, sext, u0, uext, w0, wext, qwe, qsh;
 	int lo_mask, hi_sh;  /* pair index split: fast index bits, shift */
 };
 
+/* T: the workgroup's LDS copy of a term; every field is read as a
+ * wave-uniform value */
+template <int ES, int GB>
 __device__ __forceinline__ Step
 make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
     int B0, int C0, int tcw)
 {
 	Step s;
-	s.S = (const char *)p.buf[T.s_buf];
-	s.Q = (const char *)p.buf[T.q_buf];
-	s.role = T.role;
-	s.set = T.set;
-	s.K = T.K;
-	s.neg = T.sign < 0;
-	s.packed = T.packed;
-	s.ufast = T.ufast;
-	s.s_st = T.s_st;
-	s.q_k = T.q_k;
+	s.role = uni(T.role);
+	s.set = uni(T.set);
+	s.K = uni(T.K);
+	s.neg = uni(T.sign) < 0;
+	s.packed = uni(T.packed);
+	s.ufast = uni(T.ufast);
+	s.s_st = uni64(T.s_st);
+	s.qgs = uni64(T.q_k) * GB;
 	const int na = p.na, nc = p.nc;
 	int ur, wr;
-	if (T.role == PT_ROLE_A) {
+	if (s.role == PT_ROLE_A) {
 		s.s0 = A0; s.sn = 32; s.sext = na;
 		s.u0 = B0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
 		ur = PT_ROLE_B; wr = PT_ROLE_C;
-	} else if (T.role == PT_ROLE_B) {
+	} else if (s.role == PT_ROLE_B) {
 		s.s0 = B0; s.sn = 32; s.sext = na;
 		s.u0 = A0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
 		ur = PT_ROLE_A; wr = PT_ROLE_C;
@@ -247,9 +273,15 @@ make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
 	s.qsh = s.qwe == 32 ? 5 : 4;
 	s.lo_mask = s.ufast ? 31 : s.qwe - 1;
 	s.hi_sh = s.ufast ? 5 : s.qsh;
-	s.q_u = T.q_st[ur];
-	s.q_w = T.q_st[wr];
-	s.S = (const char *)s.S; /* offsets applied by the caller's type */
+	s.q_u = uni64(T.q_st[ur]);
+	s.q_w = uni64(T.q_st[wr]);
+	const int64_t soff = uni64(T.s_off[0]) * i + uni64(T.s_off[1]) * j +
+	    uni64(T.s_off[2]) * k;
+	const int64_t qoff = uni64(T.q_off[0]) * i + uni64(T.q_off[1]) * j +
+	    uni64(T.q_off[2]) * k;
+	s.sb0 = (const char *)uni64((int64_t)p.buf[uni(T.s_buf)]) + soff * ES;
+	s.qb0 = (const char *)uni64((int64_t)p.buf[uni(T.q_buf)]) + qoff;
+	s.sbytes = (int64_t)s.sext * s.s_st * ES;
 	return s;
 }
 
@@ -299,8 +331,8 @@ typedef uint32_t v4u __attribute__((ext_vector_type(4)));
  */
 template <int F, int NPC, int SEPT>
 __device__ __forceinline__ void
-load_step(const Step &s, int64_t soff, int64_t qoff, int kc, int tid,
-    v4u (&qr)[NPC], typename Form<F>::G (&sr)[SEPT])
+load_step(const Step &s, int kc, int tid, v4u (&qr)[NPC],
+    typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
 	constexpr int KC = Form<F>::KC;
@@ -311,9 +343,7 @@ load_step(const Step &s, int64_t soff, int64_t qoff, int kc, int tid,
 		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
 		const int sg = s.s0 + row;
 		const bool rv = row < s.sn && sg < s.sext;
-		const G *sb = (const G *)s.S + soff;
-		const __amdgpu_buffer_rsrc_t rs = brsrc(sb,
-		    (int64_t)s.sext * s.s_st * ES);
+		const __amdgpu_buffer_rsrc_t rs = brsrc(s.sb0, s.sbytes);
 #pragma unroll
 		for (int e = 0; e < SEPT; ++e) {
 			const int kg = kc + k0 + e;
@@ -322,9 +352,9 @@ load_step(const Step &s, int64_t soff, int64_t qoff, int kc, int tid,
 			sr[e] = neg_if(bload<G>(rs, o), s.neg);
 		}
 	}
-	/* packed Q: descriptor over the kappa block of this chunk */
-	const char *qb = (const char *)s.Q + qoff + (int64_t)(kc / KC) * s.q_k * GB;
-	const __amdgpu_buffer_rsrc_t rq = brsrc(qb, s.q_k * GB);
+	/* packed Q: descriptor over the kappa group of this chunk */
+	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)(kc / KC) * s.qgs,
+	    s.qgs);
 	const int ch = tid & (NV - 1);
 	const int npc = ((32 * s.qwe) << LNV) >> 8;
 #pragma unroll
@@ -434,11 +464,11 @@ qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W], int gb)
 
 template <int GB, int KC>
 __device__ __forceinline__ void
-issue_q(const Step &s, int64_t qoff, int kc, char *sQ, int wave,
+issue_q(const Step &s, int kc, char *sQ, int wave,
     const uint32_t (&vo)[PT_QRUN_W])
 {
-	const char *qb = (const char *)s.Q + qoff + (int64_t)(kc / KC) * s.q_k * GB;
-	const __amdgpu_buffer_rsrc_t rq = brsrc(qb, s.q_k * GB);
+	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)(kc / KC) * s.qgs,
+	    s.qgs);
 #pragma unroll
 	for (int n = 0; n < PT_QRUN_W; ++n) {
 		const int run = wave + 4 * n;
@@ -456,25 +486,36 @@ wait_vm()
 	__asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
-/* S rows of one step: SEPT kappa per thread (sign applied), in registers */
+/* S rows of one step: SEPT kappa per thread (sign applied), in registers.
+ * soff_lane (per term): byte offset of the thread's first element of the
+ * chunk from the chunk base, or PT_OOR for a row outside the tile. */
+template <int F, int SEPT>
+__device__ __forceinline__ uint32_t
+s_lane(const Step &s, int tid)
+{
+	constexpr int KC = Form<F>::KC;
+	constexpr int ES = (int)sizeof(typename Form<F>::G);
+	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+	const int sg = s.s0 + row;
+	return (row < s.sn && sg < s.sext) ?
+	    (uint32_t)(((int64_t)sg * s.s_st + k0) * ES) : PT_OOR;
+}
+
 template <int F, int SEPT>
 __device__ __forceinline__ void
-load_s(const Step &s, int64_t soff, int kc, int tid,
+load_s(const Step &s, uint32_t soff_lane, int kc, int tid,
     typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
 	constexpr int KC = Form<F>::KC;
 	constexpr int ES = (int)sizeof(G);
-	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
-	const int sg = s.s0 + row;
-	const bool rv = row < s.sn && sg < s.sext;
-	const G *sb = (const G *)s.S + soff;
-	const __amdgpu_buffer_rsrc_t rs = brsrc(sb, (int64_t)s.sext * s.s_st * ES);
+	const int k0 = (tid % (KC / SEPT)) * SEPT;
+	const __amdgpu_buffer_rsrc_t rs = brsrc(s.sb0 + (int64_t)kc * ES,
+	    s.sbytes - (int64_t)kc * ES);
 #pragma unroll
 	for (int e = 0; e < SEPT; ++e) {
-		const int kg = kc + k0 + e;
-		const uint32_t o = (rv && kg < s.K) ?
-		    (uint32_t)(((int64_t)sg * s.s_st + kg) * ES) : PT_OOR;
+		const uint32_t o = (kc + k0 + e < s.K && soff_lane != PT_OOR) ?
+		    soff_lane + e * ES : PT_OOR;
 		sr[e] = neg_if(bload<G>(rs, o), s.neg);
 	}
 }
@@ -500,36 +541,26 @@ struct Chunk {
 
 /*
  * Phase 1 (accumulator block x = c offset, D[a][b]) over one staged kappa
- * chunk:  role a: A = S[a] (fixed), B = Q[b][c = x];
- *         role b: A = Q[a][c = x],  B = S[b] (fixed).
+ * chunk:  role a (RA): A = S[a] (fixed), B = Q[b][c = x];
+ *         role b:      A = Q[a][c = x],  B = S[b] (fixed).
  */
-template <int F, int TCW>
+template <int F, int TCW, bool RA>
 __device__ __forceinline__ void
-mma_p1(typename Form<F>::ACC (&ac)[TCW], bool ra, const char *sS,
-    const char *sQ, int wa, int wb, int ln, int lg)
+mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
+    int wa, int wb, int ln, int lg)
 {
 	typedef typename Form<F>::L L;
 	typedef typename Chunk<L>::V V;
 	constexpr int NE = Chunk<L>::N;
-	const int ws = ra ? wa : wb, wu = ra ? wb : wa;
+	const int ws = RA ? wa : wb, wu = RA ? wb : wa;
 	const V f = *(const V *)(sS + (16 * ws + ln) * Stg::S_ROW + 16 * lg);
 	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + 16 * lg;
-	if (ra) {
 #pragma unroll
-		for (int x = 0; x < TCW; ++x) {
-			const V q = *(const V *)(pv + 64 * x);
+	for (int x = 0; x < TCW; ++x) {
+		const V q = *(const V *)(pv + 64 * x);
 #pragma unroll
-			for (int e = 0; e < NE; ++e)
-				ac[x] = mma(f[e], q[e], ac[x]);
-		}
-	} else {
-#pragma unroll
-		for (int x = 0; x < TCW; ++x) {
-			const V q = *(const V *)(pv + 64 * x);
-#pragma unroll
-			for (int e = 0; e < NE; ++e)
-				ac[x] = mma(q[e], f[e], ac[x]);
-		}
+		for (int e = 0; e < NE; ++e)
+			ac[x] = RA ? mma(f[e], q[e], ac[x]) : mma(q[e], f[e], ac[x]);
 	}
 }
 
@@ -658,6 +689,7 @@ tile_item(const pt_launch &p, int64_t it)
 
 	__shared__ __attribute__((aligned(16))) char smem[SMEM];
 	__shared__ double red[4];
+	__shared__ pt_term sterm[PT_MAX_TERMS];
 	char *sS = smem;
 	char *sQ = smem + FM::NP * LD::S_PB;
 	AE *sX = (AE *)smem;
@@ -686,16 +718,25 @@ tile_item(const pt_launch &p, int64_t it)
 		for (int x = 0; x < TCW; ++x)
 			acc[s][x] = ACC{0, 0, 0, 0};
 
-	/* steps = (term, kappa chunk) over all terms; phase 1 first */
-	const int nterm = p.nterm1 + p.nterm2;
-	int t = 0, kc = 0;
-	Step cur = make_step(p, p.terms[0], i, j, k, A0, B0, C0, TCW);
-	int64_t soff, qoff;
+	/* the term table, copied once into LDS: a term change then reads it
+	 * with LDS loads instead of waiting on global memory */
+	const int nterm = uni(p.nterm1 + p.nterm2);
 	{
-		const pt_term &T = p.terms[0];
-		soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
-		qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
+		const uint32_t *src = (const uint32_t *)p.terms;
+		uint32_t *dst = (uint32_t *)sterm;
+		const int nw = nterm * (int)(sizeof(pt_term) / 4);
+		for (int w = tid; w < nw; w += PT_THREADS)
+			dst[w] = src[w];
+		__syncthreads();
 	}
+	constexpr int ES = (int)sizeof(G);
+	auto step_of = [&](int tn) {
+		return make_step<ES, GB>(p, sterm[tn], i, j, k, A0, B0, C0, TCW);
+	};
+
+	/* steps = (term, kappa chunk) over all terms; phase 1 first */
+	int t = 0, kc = 0;
+	Step cur = step_of(0);
 	G sr[SEPT];
 
 	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
@@ -727,19 +768,21 @@ tile_item(const pt_launch &p, int64_t it)
 		const int wv = __builtin_amdgcn_readfirstlane(wave);
 		uint32_t vo[PT_QRUN_W];
 		qtab(cur, wv, lane, vo, GB);
-		load_s<F, SEPT>(cur, soff, 0, tid, sr);
-		issue_q<GB, KC>(cur, qoff, 0, smem + Stg::S_BYTES, wv, vo);
+		uint32_t sl = s_lane<F, SEPT>(cur, tid);
+		load_s<F, SEPT>(cur, sl, 0, tid, sr);
+		issue_q<GB, KC>(cur, 0, smem + Stg::S_BYTES, wv, vo);
 		wait_vm();
 		store_s<F, SEPT>(tid, smem, sr);
 		__syncthreads();
 		int buf = 0;
 		/* one loop per phase, so each holds a single MFMA pattern and the
 		 * accumulators stay in place; the transpose sits between them */
+		/* phase 0 = role-a terms, 1 = role-b terms, 2 = role-c terms */
+		const int nta = uni(p.nterm_a), nt1 = uni(p.nterm1);
 		auto run_phase = [&](auto ph) {
 			constexpr int PH = decltype(ph)::value;
-			const int tend = PH == 1 ? p.nterm1 : nterm;
+			const int tend = PH == 0 ? nta : PH == 1 ? nt1 : nterm;
 			while (t < tend) {
-				const bool ra = cur.role == PT_ROLE_A;
 				const int set = cur.set;
 				int tn = t, kn = kc + KC;
 				if (kn >= cur.K) {
@@ -750,25 +793,21 @@ tile_item(const pt_launch &p, int64_t it)
 				char *st_nxt = smem + (buf ^ 1) * STAGE;
 				if (tn < nterm) {
 					if (tn != t) {
-						const pt_term &T = p.terms[tn];
-						cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
-						soff = T.s_off[0] * i + T.s_off[1] * j +
-						    T.s_off[2] * k;
-						qoff = T.q_off[0] * i + T.q_off[1] * j +
-						    T.q_off[2] * k;
+						cur = step_of(tn);
 						qtab(cur, wv, lane, vo, GB);
+						sl = s_lane<F, SEPT>(cur, tid);
 					}
-					load_s<F, SEPT>(cur, soff, kn, tid, sr);
-					issue_q<GB, KC>(cur, qoff, kn, st_nxt + Stg::S_BYTES, wv,
-					    vo);
+					load_s<F, SEPT>(cur, sl, kn, tid, sr);
+					issue_q<GB, KC>(cur, kn, st_nxt + Stg::S_BYTES, wv, vo);
 				}
 				const char *cS = st_cur, *cQ = st_cur + Stg::S_BYTES;
-				if constexpr (PH == 1) {
+				if constexpr (PH < 2) {
 					if (NSET == 1 || set == 0)
-						mma_p1<F, TCW>(acc[0], ra, cS, cQ, wa, wb, ln, lg);
+						mma_p1<F, TCW, PH == 0>(acc[0], cS, cQ, wa, wb, ln,
+						    lg);
 					else
-						mma_p1<F, TCW>(acc[NSET - 1], ra, cS, cQ, wa, wb,
-						    ln, lg);
+						mma_p1<F, TCW, PH == 0>(acc[NSET - 1], cS, cQ, wa,
+						    wb, ln, lg);
 				} else {
 					if (NSET == 1 || set == 0)
 						mma_p2<F, TCW>(acc[0], cS, cQ, wa, wb, ln, lg);
@@ -786,13 +825,14 @@ tile_item(const pt_launch &p, int64_t it)
 				buf ^= 1;
 			}
 		};
+		run_phase(std::integral_constant<int, 0>());
 		run_phase(std::integral_constant<int, 1>());
 		/* the stage consumed last is free: transpose through it */
 		transpose((AE *)(smem + (buf ^ 1) * STAGE));
 		run_phase(std::integral_constant<int, 2>());
 	} else {
 		v4u qr[NPC];
-		load_step<F, NPC, SEPT>(cur, soff, qoff, 0, tid, qr, sr);
+		load_step<F, NPC, SEPT>(cur, 0, tid, qr, sr);
 		while (t < nterm) {
 			if (t == p.nterm1 && kc == 0)
 				transpose(sX);
@@ -807,13 +847,9 @@ tile_item(const pt_launch &p, int64_t it)
 				kn = 0;
 			}
 			if (tn < nterm) {
-				if (tn != t) {
-					const pt_term &T = p.terms[tn];
-					cur = make_step(p, T, i, j, k, A0, B0, C0, TCW);
-					soff = T.s_off[0] * i + T.s_off[1] * j + T.s_off[2] * k;
-					qoff = T.q_off[0] * i + T.q_off[1] * j + T.q_off[2] * k;
-				}
-				load_step<F, NPC, SEPT>(cur, soff, qoff, kn, tid, qr, sr);
+				if (tn != t)
+					cur = step_of(tn);
+				load_step<F, NPC, SEPT>(cur, kn, tid, qr, sr);
 			}
 			if (act) {
 				if (NSET == 1 || set == 0)
@@ -1130,38 +1166,77 @@ launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
 	return (int)hipGetLastError();
 }
 
-#ifndef PT_KERNEL_F64X2
+/* Kernels built in pt_kernel_agpr.hip, the same source compiled without
+ * the VGPR-form MFMA option: FP64 with two W sets (its accumulators alone
+ * fill 256 registers, and that option gives wrong energies there) and the
+ * split forms (whose accumulators stay in place either way, and for which
+ * the option can crash the compiler). */
+constexpr bool
+agpr_unit(int F, int NSET)
+{
+	return (F == PT_FORM_FP64 && NSET == 2) || F >= PT_FORM_BF16X3;
+}
+
 template <int F>
 int
 launch_form(const pt_launch *p, hipStream_t s)
 {
-	if (p->nset == 1)
-		return launch_kind<F, 1>(p, s);
-	/* FP64 with two W sets is built in a translation unit of its own,
-	 * without the VGPR-form MFMA option (its accumulators alone fill 256
-	 * registers, and that option gives wrong results there) */
-	if constexpr (F == PT_FORM_FP64)
-		return pt_launch_tiles_f64x2(p, s);
-	else
-		return launch_kind<F, 2>(p, s);
+	if (p->nset == 1) {
+		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
+			return launch_kind<F, 1>(p, s);
+	} else {
+		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
+			return launch_kind<F, 2>(p, s);
+	}
+	return (int)hipErrorInvalidValue;
 }
-#endif
+
+template <int F>
+int
+launch_batch_form(int nset, const pt_launch *segs, const int64_t *begin,
+    int nseg, int64_t total, hipStream_t s)
+{
+	if (nset == 1) {
+		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
+			return launch_batch<F, 1>(segs, begin, nseg, total, s);
+	} else {
+		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
+			return launch_batch<F, 2>(segs, begin, nseg, total, s);
+	}
+	return (int)hipErrorInvalidValue;
+}
 
 } /* namespace */
 
-#ifdef PT_KERNEL_F64X2
+#if PT_KERNEL_AGPR
 extern "C" int
-pt_launch_tiles_f64x2(const struct pt_launch *p, void *stream)
+pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
 {
-	return launch_kind<PT_FORM_FP64, 2>(p, (hipStream_t)stream);
+	hipStream_t s = (hipStream_t)stream;
+	switch (form) {
+	case PT_FORM_FP64: return launch_kind<PT_FORM_FP64, 2>(p, s);
+	case PT_FORM_BF16X3: return launch_form<PT_FORM_BF16X3>(p, s);
+	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
+	}
+	return (int)hipErrorInvalidValue;
 }
 
 extern "C" int
-pt_launch_batch_f64x2(const struct pt_launch *segs, const int64_t *begin,
-    int nseg, int64_t total, void *stream)
+pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
+    const int64_t *begin, int nseg, int64_t total, void *stream)
 {
-	return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
-	    (hipStream_t)stream);
+	hipStream_t s = (hipStream_t)stream;
+	switch (form) {
+	case PT_FORM_FP64:
+		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total, s);
+	case PT_FORM_BF16X3:
+		return launch_batch_form<PT_FORM_BF16X3>(nset, segs, begin, nseg,
+		    total, s);
+	case PT_FORM_F16X3:
+		return launch_batch_form<PT_FORM_F16X3>(nset, segs, begin, nseg,
+		    total, s);
+	}
+	return (int)hipErrorInvalidValue;
 }
 #else
 extern "C" int
@@ -1170,11 +1245,11 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 	hipStream_t s = (hipStream_t)stream;
 	if (p->n_chunk <= 0)
 		return 0;
+	if (agpr_unit(form, p->nset))
+		return pt_launch_tiles_agpr(form, p, stream);
 	switch (form) {
 	case PT_FORM_FP64: return launch_form<PT_FORM_FP64>(p, s);
 	case PT_FORM_FP32: return launch_form<PT_FORM_FP32>(p, s);
-	case PT_FORM_BF16X3: return launch_form<PT_FORM_BF16X3>(p, s);
-	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1186,16 +1261,16 @@ pt_launch_batch(int form, int nset, const struct pt_launch *segs,
 	hipStream_t s = (hipStream_t)stream;
 	if (nseg <= 0 || total <= 0)
 		return 0;
-	if (nset == 2 && form == PT_FORM_FP64)
-		return pt_launch_batch_f64x2(segs, begin, nseg, total, stream);
-	switch (form * 2 + (nset - 1)) {
-	case 0: return launch_batch<PT_FORM_FP64, 1>(segs, begin, nseg, total, s);
-	case 2: return launch_batch<PT_FORM_FP32, 1>(segs, begin, nseg, total, s);
-	case 3: return launch_batch<PT_FORM_FP32, 2>(segs, begin, nseg, total, s);
-	case 4: return launch_batch<PT_FORM_BF16X3, 1>(segs, begin, nseg, total, s);
-	case 5: return launch_batch<PT_FORM_BF16X3, 2>(segs, begin, nseg, total, s);
-	case 6: return launch_batch<PT_FORM_F16X3, 1>(segs, begin, nseg, total, s);
-	case 7: return launch_batch<PT_FORM_F16X3, 2>(segs, begin, nseg, total, s);
+	if (agpr_unit(form, nset))
+		return pt_launch_batch_agpr(form, nset, segs, begin, nseg, total,
+		    stream);
+	switch (form) {
+	case PT_FORM_FP64:
+		return launch_batch_form<PT_FORM_FP64>(nset, segs, begin, nseg,
+		    total, s);
+	case PT_FORM_FP32:
+		return launch_batch_form<PT_FORM_FP32>(nset, segs, begin, nseg,
+		    total, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1285,4 +1360,4 @@ pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n, int fp32,
 	    fp32);
 	return (int)hipGetLastError();
 }
-#endif /* PT_KERNEL_F64X2 */
+#endif /* PT_KERNEL_AGPR */

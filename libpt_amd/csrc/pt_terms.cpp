@@ -189,13 +189,17 @@ pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
 }
 
 int
-pt_terms_order(std::vector<pt_term> &terms)
+pt_terms_order(std::vector<pt_term> &terms, int *n_role_a)
 {
-	/* phase 1: single index a or b; phase 2: single index c */
-	std::vector<pt_term> p1, p2;
+	/* phase 1: single index a, then single index b; phase 2: single
+	 * index c.  Each group keeps the table order. */
+	std::vector<pt_term> ga, gb, gc;
 	for (const pt_term &t : terms)
-		(t.role == PT_ROLE_C ? p2 : p1).push_back(t);
-	terms = p1;
-	terms.insert(terms.end(), p2.begin(), p2.end());
-	return (int)p1.size();
+		(t.role == PT_ROLE_A ? ga : t.role == PT_ROLE_B ? gb : gc).push_back(t);
+	terms = ga;
+	terms.insert(terms.end(), gb.begin(), gb.end());
+	terms.insert(terms.end(), gc.begin(), gc.end());
+	if (n_role_a)
+		*n_role_a = (int)ga.size();
+	return (int)(ga.size() + gb.size());
 }
