@@ -32,18 +32,28 @@ enum pt_role { PT_ROLE_A = 0, PT_ROLE_B = 1, PT_ROLE_C = 2 };
 #define PT_MAX_TERMS 40
 #define PT_MAX_BUFS 24
 
-/* kappa block of the packed Q operands per form (one block of one (u, w)
- * pair is 64 bytes: 8 FP64, 16 FP32, or 32 hi + 32 lo halves = 128 bytes) */
+/* kappa per group of the packed Q operands: one group of one (u, w) pair
+ * is 32 bytes for FP64 (4 kappa) and FP32 (8 kappa), 128 bytes for the
+ * split forms (32 hi halves then 32 lo halves) */
 static inline int
 pt_kc(int form)
 {
-	return form == 0 ? 8 : form == 1 ? 16 : 32;
+	return form == 0 ? 4 : form == 1 ? 8 : 32;
 }
 
 static inline int
 pt_group_bytes(int form)
 {
-	return form >= 2 ? 128 : 64;
+	return form >= 2 ? 128 : 32;
+}
+
+/* groups per packed slab of K kappa (zero padded); FP64 / FP32 round up
+ * to an even count, since a 16-wide tile step takes two groups */
+static inline int64_t
+pt_kgroups(int form, int64_t K)
+{
+	const int64_t kc = pt_kc(form);
+	return form >= 2 ? (K + kc - 1) / kc : 2 * ((K + 2 * kc - 1) / (2 * kc));
 }
 
 /* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
@@ -134,7 +144,7 @@ int pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
 /* [R][K][PL] -> [R][ceil(K/KC)][PL][KC] (mode 0 FP64, 1 FP32) or
  * [R][ceil(K/KC)][PL][hi KC][lo KC] halves (mode 2 BF16, 3 FP16, of x*scale) */
 int pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K,
-    int64_t PL, int KC, int mode, double scale, void *stream);
+    int64_t nkb, int64_t PL, int KC, int mode, double scale, void *stream);
 int pt_launch_absmax(const double *src, int64_t n, double *out, void *stream);
 int pt_launch_copy(const double *src, double *dst, int64_t n, void *stream);
 int pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n,

@@ -320,10 +320,10 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 			pt_terms_mixed(c.terms, s, oa, va, ob, vb);
 	}
 	/* Q slab offsets: API elements (K q_k per slab) -> bytes of the
-	 * packed copy (ceil(K / KC) q_k groups per slab) */
+	 * packed copy (pt_kgroups q_k groups per slab) */
 	for (pt_term &t : c.terms) {
 		const int64_t slab = (int64_t)t.K * t.q_k;
-		const int64_t pslab = ((t.K + pt_kc(form) - 1) / pt_kc(form)) * t.q_k *
+		const int64_t pslab = pt_kgroups(form, t.K) * t.q_k *
 		    pt_group_bytes(form);
 		for (int x = 0; x < 3; ++x)
 			t.q_off[x] = t.q_off[x] / slab * pslab;
@@ -495,13 +495,14 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 			Blocks &dstb = fam == 0 ? qT[s] : qV[s];
 			for (int b = 0; b < nblk; ++b) {
 				const Geo g = geo(fam, b);
-				const int64_t nkb = (g.K + KC - 1) / KC;
+				const int64_t nkb = pt_kgroups(form, g.K);
 				void *dst;
 				PFAIL(dev_alloc(&dst, (size_t)(g.R * nkb * g.PL) * GB));
 				pl->owned.push_back(dst);
 				const char *src = (const char *)arr[api] + esz * (size_t)g.off;
-				PFAIL(pt_launch_pack(src, dst, g.R, g.K, g.PL, KC, form, sc,
-				    pl->stream) == 0 || (set_err("pack launch failed"), false));
+				PFAIL(pt_launch_pack(src, dst, g.R, g.K, nkb, g.PL, KC, form,
+				    sc, pl->stream) == 0 ||
+				    (set_err("pack launch failed"), false));
 				dstb.p[b] = (const char *)dst;
 			}
 			if (restricted) {

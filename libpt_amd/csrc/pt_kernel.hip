@@ -417,147 +417,153 @@ store_step(const Step &s, int tid, char *sS, char *sQ, const v4u (&qr)[NPC],
 }
 
 /*
- * FP64 / FP32 stage (bytes):
- *   S: 32 rows x 80 B (one kappa chunk of 64 B plus 16 B of padding, so the
- *      16 rows read by one MFMA operand fall into distinct bank groups);
- *   Q: 32 u x (32 w x 64 B + 16 B): piece (u, w, ch) at u * 2064 + w * 64
- *      + 16 ch.  The 16 consecutive u of one operand read land in distinct
- *      16-byte bank groups, and consecutive w are 64 B apart, so every
- *      operand read of a step is one base register plus an immediate.
- * Lane group lg of an MFMA takes 16-byte chunk lg of a row, i.e. kappa
- * 2lg, 2lg+1 (FP64) or 4lg .. 4lg+3 (FP32), for both operands; one LDS
- * read therefore feeds 2 (FP64) or 4 (FP32) MFMAs.
+ * FP64 / FP32 staging: NSLOT slots, each holding one step (term, kappa
+ * chunk) of the tile, loaded NSLOT - 1 steps ahead of the step computed.
+ * Slot (bytes):
+ *   S: 32 rows x 80 B (the chunk's kappa of one row, then padding, so the
+ *      16 rows one MFMA operand read touches fall in distinct bank groups);
+ *   Q: 32 u-rows x 1040 B.  A u-row is the chunk of the tile's w range:
+ *      type A (w range 16, roles a and b of a 16-wide c tile): 16 w x 64 B,
+ *        i.e. two 32-byte packed kappa groups per (u, w);
+ *      type B (w range 32: role c, and 32-wide c tiles): 32 w x 32 B, one
+ *        packed group per (u, w).
+ *      Either way a u-row is one 1 KB DMA run (lane l writes bytes 16 l ..
+ *      16 l + 15), and the 16-byte pad puts consecutive u in distinct bank
+ *      groups.
+ * MFMA lane group lg reads chunk lg of a type-A row (16 B: kappa 2lg, 2lg+1
+ * for FP64, 4lg .. 4lg+3 for FP32) or element lg of a type-B row (8 B: kappa
+ * lg for FP64, 2lg, 2lg+1 for FP32), the same mapping for both operands.
  */
 struct Stg {
 	static constexpr int S_ROW = 80;
 	static constexpr int S_BYTES = 32 * S_ROW;
-	static constexpr int Q_USTR = 32 * 64 + 16;
+	static constexpr int Q_USTR = 1024 + 16;
 	static constexpr int Q_BYTES = 32 * Q_USTR;
-	static constexpr int STAGE = S_BYTES + Q_BYTES;
-	static constexpr int BYTES = 2 * STAGE;
+	static constexpr int SLOT = S_BYTES + Q_BYTES;
+	static constexpr int NSLOT = 2;
+	static constexpr int BYTES = NSLOT * SLOT;
 };
 
-/*
- * Q is streamed HBM -> LDS without registers (buffer_load ... lds): one
- * wave instruction fills 1 KB of LDS (lane l at +16 l), i.e. 16 w rows of
- * one u.  The 64 runs (u = run % 32, w half = run / 32) are dealt to the
- * waves round-robin.  Lane offsets depend on the term only (not on kappa),
- * so they are computed once per term; each step only moves the descriptor.
- */
-#define PT_QRUN_W 16  /* runs per wave */
+#define PT_QRUN_W 8   /* DMA runs per wave per step (32 u-rows / 4 waves) */
 
+/* lane offsets of the Q runs of one term (type A: w = lane / 4, 16-byte
+ * chunk c = lane % 4 from group c / 2; type B: w = lane / 2, chunk
+ * lane % 2); gstride = bytes per packed group of the slab */
+template <bool TA>
 __device__ __forceinline__ void
-qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W], int gb)
+qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W])
 {
+	const int w = TA ? (lane >> 2) : (lane >> 1);
+	const uint32_t cofs = TA ? (uint32_t)((lane & 1) * 16 +
+	    ((lane >> 1) & 1) * s.qgs) : (uint32_t)((lane & 1) * 16);
 #pragma unroll
 	for (int n = 0; n < PT_QRUN_W; ++n) {
-		const int run = wave + 4 * n;
-		const int u = run & 31, w = 16 * (run >> 5) + (lane >> 2);
-		const int ug = s.u0 + u, wg = s.w0 + w;
-		const bool v = w < s.qwe && ug < s.uext && wg < s.wext &&
-		    (!s.packed || ug > wg);
+		const int ug = s.u0 + wave + 4 * n, wg = s.w0 + w;
+		const bool v = ug < s.uext && wg < s.wext && (!s.packed || ug > wg);
 		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
 		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
-		vo[n] = v ? (uint32_t)(pidx * gb + (lane & 3) * 16) : PT_OOR;
+		vo[n] = v ? (uint32_t)(pidx * 32) + cofs : PT_OOR;
 	}
 }
 
-template <int GB, int KC>
+/* DMA of step g (kappa groups g or 2g, 2g + 1) of a term into a slot */
+template <bool TA>
 __device__ __forceinline__ void
-issue_q(const Step &s, int kc, char *sQ, int wave,
+issue_q(const Step &s, int g, char *sQ, int wave,
     const uint32_t (&vo)[PT_QRUN_W])
 {
-	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)(kc / KC) * s.qgs,
-	    s.qgs);
+	constexpr int NG = TA ? 2 : 1;
+	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)g * NG * s.qgs,
+	    NG * s.qgs);
 #pragma unroll
-	for (int n = 0; n < PT_QRUN_W; ++n) {
-		const int run = wave + 4 * n;
-		const int h = run >> 5;
-		if (16 * h < s.qwe)
-			__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
-			    (__attribute__((address_space(3))) void *)(sQ +
-			    (run & 31) * Stg::Q_USTR + 1024 * h), 16, vo[n], 0, 0, 0);
-	}
+	for (int n = 0; n < PT_QRUN_W; ++n)
+		__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
+		    (__attribute__((address_space(3))) void *)(sQ +
+		    (wave + 4 * n) * Stg::Q_USTR), 16, vo[n], 0, 0, 0);
 }
 
+/* wait until at most n vector-memory loads of this wave are outstanding */
+template <int N>
 __device__ __forceinline__ void
 wait_vm()
 {
-	__asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+	__asm__ volatile("s_waitcnt vmcnt(%0)" :: "n"(N) : "memory");
 }
 
-/* S rows of one step: SEPT kappa per thread (sign applied), in registers.
- * soff_lane (per term): byte offset of the thread's first element of the
- * chunk from the chunk base, or PT_OOR for a row outside the tile. */
+/* S loads: thread -> (row, kappa k0 .. k0 + SEPT - 1) of a chunk of KS
+ * kappa, KS / SEPT threads per row (8 for type A, 4 for type B); rows
+ * beyond the tile load zeros and are not stored.  Per term: the byte
+ * offset from the chunk base (or PT_OOR) and the LDS store offset (or -1). */
 template <int F, int SEPT>
-__device__ __forceinline__ uint32_t
-s_lane(const Step &s, int tid)
+__device__ __forceinline__ void
+s_lane(const Step &s, int tid, bool ta, uint32_t &ofs, int &lofs)
 {
-	constexpr int KC = Form<F>::KC;
 	constexpr int ES = (int)sizeof(typename Form<F>::G);
-	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+	const int tpr = ta ? 8 : 4;
+	const int row = tid / tpr, k0 = (tid % tpr) * SEPT;
 	const int sg = s.s0 + row;
-	return (row < s.sn && sg < s.sext) ?
+	ofs = (row < s.sn && sg < s.sext) ?
 	    (uint32_t)(((int64_t)sg * s.s_st + k0) * ES) : PT_OOR;
+	lofs = row < 32 ? row * Stg::S_ROW + k0 * ES : -1;
 }
 
 template <int F, int SEPT>
 __device__ __forceinline__ void
-load_s(const Step &s, uint32_t soff_lane, int kc, int tid,
+load_s(const Step &s, uint32_t ofs, int k0, int kc, int ks,
     typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
-	constexpr int KC = Form<F>::KC;
 	constexpr int ES = (int)sizeof(G);
-	const int k0 = (tid % (KC / SEPT)) * SEPT;
 	const __amdgpu_buffer_rsrc_t rs = brsrc(s.sb0 + (int64_t)kc * ES,
 	    s.sbytes - (int64_t)kc * ES);
 #pragma unroll
 	for (int e = 0; e < SEPT; ++e) {
-		const uint32_t o = (kc + k0 + e < s.K && soff_lane != PT_OOR) ?
-		    soff_lane + e * ES : PT_OOR;
+		const uint32_t o = (k0 + e < ks && kc + k0 + e < s.K &&
+		    ofs != PT_OOR) ? ofs + e * ES : PT_OOR;
 		sr[e] = neg_if(bload<G>(rs, o), s.neg);
 	}
 }
 
 template <int F, int SEPT>
 __device__ __forceinline__ void
-store_s(int tid, char *sS, const typename Form<F>::G (&sr)[SEPT])
+store_s(char *sS, int lofs, const typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
-	constexpr int KC = Form<F>::KC;
-	const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
+	if (lofs >= 0) {
 #pragma unroll
-	for (int e = 0; e < SEPT; ++e)
-		*(G *)(sS + row * Stg::S_ROW + (k0 + e) * (int)sizeof(G)) = sr[e];
+		for (int e = 0; e < SEPT; ++e)
+			*(G *)(sS + lofs + e * (int)sizeof(G)) = sr[e];
+	}
 }
 
-/* element e of a 16-byte chunk as an MFMA operand */
-template <class L>
+/* an MFMA operand chunk of NB bytes (16: type A, 8: type B) */
+template <class L, int NB>
 struct Chunk {
-	typedef L V __attribute__((ext_vector_type(16 / sizeof(L))));
-	static constexpr int N = 16 / sizeof(L);
+	typedef L V __attribute__((ext_vector_type(NB / sizeof(L))));
+	static constexpr int N = NB / sizeof(L);
 };
 
 /*
- * Phase 1 (accumulator block x = c offset, D[a][b]) over one staged kappa
- * chunk:  role a (RA): A = S[a] (fixed), B = Q[b][c = x];
- *         role b:      A = Q[a][c = x],  B = S[b] (fixed).
+ * Phase 1 (accumulator block x = c offset, D[a][b]) over one staged step:
+ *   role a (RA): A = S[a] (fixed), B = Q[b][c = x];
+ *   role b:      A = Q[a][c = x],  B = S[b] (fixed).
+ * TA: type-A rows (64 B per w), else type-B rows (32 B per w).
  */
-template <int F, int TCW, bool RA>
+template <int F, int TCW, bool RA, bool TA>
 __device__ __forceinline__ void
 mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
     int wa, int wb, int ln, int lg)
 {
 	typedef typename Form<F>::L L;
-	typedef typename Chunk<L>::V V;
-	constexpr int NE = Chunk<L>::N;
+	constexpr int NB = TA ? 16 : 8, WB = TA ? 64 : 32;
+	typedef typename Chunk<L, NB>::V V;
+	constexpr int NE = Chunk<L, NB>::N;
 	const int ws = RA ? wa : wb, wu = RA ? wb : wa;
-	const V f = *(const V *)(sS + (16 * ws + ln) * Stg::S_ROW + 16 * lg);
-	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + 16 * lg;
+	const V f = *(const V *)(sS + (16 * ws + ln) * Stg::S_ROW + NB * lg);
+	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + NB * lg;
 #pragma unroll
 	for (int x = 0; x < TCW; ++x) {
-		const V q = *(const V *)(pv + 64 * x);
+		const V q = *(const V *)(pv + WB * x);
 #pragma unroll
 		for (int e = 0; e < NE; ++e)
 			ac[x] = RA ? mma(f[e], q[e], ac[x]) : mma(q[e], f[e], ac[x]);
@@ -565,7 +571,8 @@ mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 }
 
 /*
- * Phase 2 (block x = 16 cb + b, D[a][c]): role c: A = Q[a][b], B = S[c].
+ * Phase 2 (block x = 16 cb + b, D[a][c]): role c: A = Q[a][b], B = S[c];
+ * always type-B rows.
  */
 template <int F, int TCW>
 __device__ __forceinline__ void
@@ -573,17 +580,16 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
     int wa, int wb, int ln, int lg)
 {
 	typedef typename Form<F>::L L;
-	typedef typename Chunk<L>::V V;
-	constexpr int NE = Chunk<L>::N, NCB = TCW / 16;
+	typedef typename Chunk<L, 8>::V V;
+	constexpr int NE = Chunk<L, 8>::N, NCB = TCW / 16;
 	V f[NCB];
 #pragma unroll
 	for (int cb = 0; cb < NCB; ++cb)
-		f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 16 * lg);
-	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR + 16 * wb * 64 +
-	    16 * lg;
+		f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
+	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR + 16 * wb * 32 + 8 * lg;
 #pragma unroll
 	for (int b = 0; b < 16; ++b) {
-		const V q = *(const V *)(pv + 64 * b);
+		const V q = *(const V *)(pv + 32 * b);
 #pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
@@ -655,9 +661,12 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
 
 /* -------------------------------------------------------------- kernel */
 
-/* workgroups per CU the register allocation targets */
+/* workgroups per CU the register allocation targets: two where one
+ * workgroup's LDS (two slots, 76 KB) and registers (<= 256 per lane) allow,
+ * so that one workgroup's staging and barriers overlap the other's MFMAs */
 template <int F, int NSET> struct Occ {
-	static constexpr int WG_PER_CU = 1;
+	static constexpr int WG_PER_CU =
+	    (F == PT_FORM_FP64 || F == PT_FORM_FP32) && NSET == 1 ? 2 : 1;
 };
 
 /*
@@ -680,9 +689,13 @@ tile_item(const pt_launch &p, int64_t it)
 	constexpr int KC = FM::KC;
 	constexpr int NCB = (F == PT_FORM_FP32 && NSET == 1) ? 2 : 1; /* pt_tcw */
 	constexpr int TCW = 16 * NCB;
-	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
-	constexpr int NPC = 32 * 32 * (GB / 16) / PT_THREADS; /* Q pieces/thread */
-	constexpr int SEPT = 32 * KC / PT_THREADS; /* S elements per thread */
+	/* packed group bytes: 128 (split), 32 (FP64 / FP32) */
+	constexpr int GB = FM::SPLIT ? 128 : 32;
+	constexpr int NPC = FM::SPLIT ? 32 * 32 * (GB / 16) / PT_THREADS : 1;
+	/* S elements per thread (split: per 32-kappa chunk; FP64 / FP32: per
+	 * step of either type) */
+	constexpr int SEPT = FM::SPLIT ? 32 * KC / PT_THREADS :
+	    (F == PT_FORM_FP64 ? 1 : 2);
 	constexpr int SMEM = FM::SPLIT ? LD::BYTES : Stg::BYTES;
 	static_assert(SMEM >= 4 * NSET * TCW * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
@@ -737,7 +750,6 @@ tile_item(const pt_launch &p, int64_t it)
 	/* steps = (term, kappa chunk) over all terms; phase 1 first */
 	int t = 0, kc = 0;
 	Step cur = step_of(0);
-	G sr[SEPT];
 
 	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
 	auto transpose = [&](AE *scratch) {
@@ -761,77 +773,186 @@ tile_item(const pt_launch &p, int64_t it)
 	};
 
 	if constexpr (!FM::SPLIT) {
-		/* double-buffered stages; Q streamed straight into LDS, S through
-		 * registers (sign applied); the stage of step n + 1 loads while
-		 * step n computes */
-		constexpr int STAGE = Stg::STAGE;
+		/*
+		 * Steps n = 0 .. N-1 walk the terms (role a, role b, role c) and
+		 * their kappa chunks; step n lives in slot n % NSLOT.  The loads of
+		 * step n + D (D = NSLOT - 1: Q by DMA, S into registers) are issued
+		 * while step n computes; S values wait in a register ring and are
+		 * written to their slot one step before use.  One barrier per step.
+		 */
+		constexpr int NS = Stg::NSLOT, D = NS - 1;
+		constexpr int KSA = KC, KSB = KC / 2;   /* kappa per step, type A / B */
+		constexpr int VMS = PT_QRUN_W + SEPT;   /* loads per step per wave */
 		const int wv = __builtin_amdgcn_readfirstlane(wave);
-		uint32_t vo[PT_QRUN_W];
-		qtab(cur, wv, lane, vo, GB);
-		uint32_t sl = s_lane<F, SEPT>(cur, tid);
-		load_s<F, SEPT>(cur, sl, 0, tid, sr);
-		issue_q<GB, KC>(cur, 0, smem + Stg::S_BYTES, wv, vo);
-		wait_vm();
-		store_s<F, SEPT>(tid, smem, sr);
-		__syncthreads();
-		int buf = 0;
-		/* one loop per phase, so each holds a single MFMA pattern and the
-		 * accumulators stay in place; the transpose sits between them */
-		/* phase 0 = role-a terms, 1 = role-b terms, 2 = role-c terms */
 		const int nta = uni(p.nterm_a), nt1 = uni(p.nterm1);
+		/* type A: a 16-wide w range (roles a and b of a 16-wide c tile) */
+		auto type_a = [&](int role) { return TCW == 16 && role != PT_ROLE_C; };
+
+		/* prefetch walker: term pt, chunk pk of step pn (the next to issue) */
+		int pt = 0, pk = 0, pn = 0;
+		Step pre = cur;
+		bool pa = type_a(pre.role);
+		uint32_t vo[PT_QRUN_W], sofs;
+		int slofs;
+		auto term_tables = [&]() {
+			if (pa)
+				qtab<true>(pre, wv, lane, vo);
+			else
+				qtab<false>(pre, wv, lane, vo);
+			s_lane<F, SEPT>(pre, tid, pa, sofs, slofs);
+		};
+		term_tables();
+		/* register ring: R[m] = S of step n + 1 + m (m < D - 1), its LDS
+		 * offset LR[m]; the loads of step n + D land in RN / LN */
+		constexpr int DR = D > 1 ? D - 1 : 1;
+		G R[DR][SEPT], RN[SEPT];
+		int LR[DR], LN = -1;
+#pragma unroll
+		for (int m = 0; m < DR; ++m)
+			LR[m] = -1;
+		/* number of steps */
+		int N = 0;
+		for (int tt = 0; tt < nterm; ++tt) {
+			const int kk = uni(sterm[tt].K);
+			const int ks = type_a(uni(sterm[tt].role)) ? KSA : KSB;
+			N += (kk + ks - 1) / ks;
+		}
+		/* issue step pn into its slot; its S goes to RN */
+		auto issue = [&]() {
+			char *slot = smem + (pn % NS) * Stg::SLOT;
+			const int ks = pa ? KSA : KSB;
+			if (pa)
+				issue_q<true>(pre, pk / KSA, slot + Stg::S_BYTES, wv, vo);
+			else
+				issue_q<false>(pre, pk / KSB, slot + Stg::S_BYTES, wv, vo);
+			const int k0 = (tid % (pa ? 8 : 4)) * SEPT;
+			load_s<F, SEPT>(pre, sofs, k0, pk, ks, RN);
+			LN = slofs;
+			++pn;
+			pk += ks;
+			if (pk >= pre.K) {
+				pk = 0;
+				if (++pt < nterm) {
+					pre = step_of(pt);
+					pa = type_a(pre.role);
+					term_tables();
+				}
+			}
+		};
+		/* after step n: the ring moves one step on, RN joins at the end */
+		auto rotate = [&]() {
+			if constexpr (D > 1) {
+#pragma unroll
+				for (int m = 0; m + 1 < DR; ++m) {
+#pragma unroll
+					for (int e = 0; e < SEPT; ++e)
+						R[m][e] = R[m + 1][e];
+					LR[m] = LR[m + 1];
+				}
+#pragma unroll
+				for (int e = 0; e < SEPT; ++e)
+					R[DR - 1][e] = RN[e];
+				LR[DR - 1] = LN;
+			}
+		};
+		/* wait until the loads of step m (issued) are complete: at most
+		 * (pn - 1 - m) later steps' loads may stay outstanding */
+		auto wait_step = [&](int m) {
+			const int later = pn - 1 - m;
+			if (D >= 3 && later >= 2)
+				wait_vm<2 * VMS>();
+			else if (D >= 2 && later >= 1)
+				wait_vm<VMS>();
+			else
+				wait_vm<0>();
+		};
+
+		/* prologue: steps 0 .. D-1 in flight, step 0 staged; entering
+		 * iteration n the ring holds R[m] = S of step n + 1 + m */
+		{
+			G r0[SEPT];
+			issue();
+#pragma unroll
+			for (int e = 0; e < SEPT; ++e)
+				r0[e] = RN[e];
+			const int l0 = LN;
+#pragma unroll
+			for (int m = 0; m + 1 < D; ++m) {
+				if (pn < N) {
+					issue();
+#pragma unroll
+					for (int e = 0; e < SEPT; ++e)
+						R[m][e] = RN[e];
+					LR[m] = LN;
+				}
+			}
+			wait_step(0);
+			store_s<F, SEPT>(smem, l0, r0);
+			__syncthreads();
+		}
+
+		/* compute walker: term ct, chunk ck of step n */
+		int ct = 0, ck = 0, cK = uni(sterm[0].K), cset = uni(sterm[0].set);
+		int n = 0;
 		auto run_phase = [&](auto ph) {
 			constexpr int PH = decltype(ph)::value;
+			constexpr bool TA = PH < 2 && TCW == 16;
 			const int tend = PH == 0 ? nta : PH == 1 ? nt1 : nterm;
-			while (t < tend) {
-				const int set = cur.set;
-				int tn = t, kn = kc + KC;
-				if (kn >= cur.K) {
-					tn = t + 1;
-					kn = 0;
+			while (ct < tend) {
+				/* loads of step n + D into the slot step n - 1 used */
+				if (pn < N) {
+					issue();
 				}
-				const char *st_cur = smem + buf * STAGE;
-				char *st_nxt = smem + (buf ^ 1) * STAGE;
-				if (tn < nterm) {
-					if (tn != t) {
-						cur = step_of(tn);
-						qtab(cur, wv, lane, vo, GB);
-						sl = s_lane<F, SEPT>(cur, tid);
-					}
-					load_s<F, SEPT>(cur, sl, kn, tid, sr);
-					issue_q<GB, KC>(cur, kn, st_nxt + Stg::S_BYTES, wv, vo);
-				}
-				const char *cS = st_cur, *cQ = st_cur + Stg::S_BYTES;
+				const char *slot = smem + (n % NS) * Stg::SLOT;
+				const char *cS = slot, *cQ = slot + Stg::S_BYTES;
 				if constexpr (PH < 2) {
-					if (NSET == 1 || set == 0)
-						mma_p1<F, TCW, PH == 0>(acc[0], cS, cQ, wa, wb, ln,
-						    lg);
+					if (NSET == 1 || cset == 0)
+						mma_p1<F, TCW, PH == 0, TA>(acc[0], cS, cQ, wa, wb,
+						    ln, lg);
 					else
-						mma_p1<F, TCW, PH == 0>(acc[NSET - 1], cS, cQ, wa,
-						    wb, ln, lg);
+						mma_p1<F, TCW, PH == 0, TA>(acc[NSET - 1], cS, cQ,
+						    wa, wb, ln, lg);
 				} else {
-					if (NSET == 1 || set == 0)
+					if (NSET == 1 || cset == 0)
 						mma_p2<F, TCW>(acc[0], cS, cQ, wa, wb, ln, lg);
 					else
 						mma_p2<F, TCW>(acc[NSET - 1], cS, cQ, wa, wb, ln,
 						    lg);
 				}
-				if (tn < nterm) {
-					wait_vm();
-					store_s<F, SEPT>(tid, st_nxt, sr);
+				/* stage the S rows of step n + 1 */
+				if (n + 1 < N) {
+					wait_step(n + 1);
+					char *sn = smem + ((n + 1) % NS) * Stg::SLOT;
+					if constexpr (D > 1)
+						store_s<F, SEPT>(sn, LR[0], R[0]);
+					else
+						store_s<F, SEPT>(sn, LN, RN);
+					rotate();
 				}
 				__syncthreads();
-				t = tn;
-				kc = kn;
-				buf ^= 1;
+				++n;
+				ck += TA ? KSA : KSB;
+				if (ck >= cK) {
+					ck = 0;
+					if (++ct < nterm) {
+						cK = uni(sterm[ct].K);
+						cset = uni(sterm[ct].set);
+					}
+				}
 			}
 		};
 		run_phase(std::integral_constant<int, 0>());
 		run_phase(std::integral_constant<int, 1>());
-		/* the stage consumed last is free: transpose through it */
-		transpose((AE *)(smem + (buf ^ 1) * STAGE));
+		/* the slot of the last phase-1 step is free until step n issues
+		 * into it: transpose through it */
+		transpose((AE *)(smem + ((n + NS - 1) % NS) * Stg::SLOT));
 		run_phase(std::integral_constant<int, 2>());
+		(void)t;
+		(void)kc;
+		(void)cur;
 	} else {
 		v4u qr[NPC];
+		G sr[SEPT];
 		load_step<F, NPC, SEPT>(cur, 0, tid, qr, sr);
 		while (t < nterm) {
 			if (t == p.nterm1 && kc == 0)
@@ -1039,10 +1160,9 @@ split2(double x, int fp16, uint16_t &hb, uint16_t &lb)
  * (split, of x * scale); kappa >= K is zero.  One thread per group: reads
  * coalesce over PL, each group is written contiguously. */
 __global__ void
-k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t PL, int KC,
-    int mode, double scale)
+k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
+    int64_t PL, int KC, int mode, double scale)
 {
-	const int64_t nkb = (K + KC - 1) / KC;
 	const int64_t total = R * nkb * PL;
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	    t < total; t += (int64_t)gridDim.x * blockDim.x) {
@@ -1316,15 +1436,15 @@ pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
 }
 
 extern "C" int
-pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t PL,
-    int KC, int mode, double scale, void *stream)
+pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
+    int64_t PL, int KC, int mode, double scale, void *stream)
 {
-	const int64_t total = R * ((K + KC - 1) / KC) * PL;
+	const int64_t total = R * nkb * PL;
 	if (total <= 0)
 		return 0;
 	int64_t nb = (total + 255) / 256;
 	k_pack<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0,
-	    (hipStream_t)stream>>>(src, dst, R, K, PL, KC, mode, scale);
+	    (hipStream_t)stream>>>(src, dst, R, K, nkb, PL, KC, mode, scale);
 	return (int)hipGetLastError();
 }
 
