@@ -33,27 +33,28 @@ enum pt_role { PT_ROLE_A = 0, PT_ROLE_B = 1, PT_ROLE_C = 2 };
 #define PT_MAX_BUFS 24
 
 /* kappa per group of the packed Q operands: one group of one (u, w) pair
- * is 32 bytes for FP64 (4 kappa) and FP32 (8 kappa), 128 bytes for the
- * split forms (32 hi halves then 32 lo halves) */
+ * is 32 bytes: 4 FP64, 8 FP32, or 8 split (hi, lo) pairs stored as
+ * hi 2q, hi 2q+1, lo 2q, lo 2q+1 for q = 0 .. 3 */
 static inline int
 pt_kc(int form)
 {
-	return form == 0 ? 4 : form == 1 ? 8 : 32;
+	return form == 0 ? 4 : 8;
 }
 
 static inline int
 pt_group_bytes(int form)
 {
-	return form >= 2 ? 128 : 32;
+	(void)form;
+	return 32;
 }
 
-/* groups per packed slab of K kappa (zero padded); FP64 / FP32 round up
- * to an even count, since a 16-wide tile step takes two groups */
+/* groups per packed slab of K kappa (zero padded), rounded up to an even
+ * count: a step over a 16-wide w range takes two groups */
 static inline int64_t
 pt_kgroups(int form, int64_t K)
 {
 	const int64_t kc = pt_kc(form);
-	return form >= 2 ? (K + kc - 1) / kc : 2 * ((K + 2 * kc - 1) / (2 * kc));
+	return 2 * ((K + 2 * kc - 1) / (2 * kc));
 }
 
 /* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.

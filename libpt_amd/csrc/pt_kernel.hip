@@ -43,9 +43,10 @@
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef _Float16 v8h __attribute__((ext_vector_type(8)));
-typedef uint16_t v8u __attribute__((ext_vector_type(8)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef _Float16 v4h __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -73,48 +74,15 @@ template <> struct Form<PT_FORM_FP32> {
 
 template <> struct Form<PT_FORM_BF16X3> {
 	typedef uint32_t G;  /* hi in bits 0-15, lo in bits 16-31 */
-	typedef uint16_t L;
+	typedef uint32_t L;  /* LDS word: two hi or two lo halves */
 	typedef float AE;
 	typedef v4f ACC;
 	typedef double ER;
-	static constexpr int NP = 2, KC = 32, KS = 32;
+	static constexpr int NP = 1, KC = 16, KS = 16;
 	static constexpr bool ROW_F64 = false, SPLIT = true;
 };
 
 template <> struct Form<PT_FORM_F16X3> : Form<PT_FORM_BF16X3> {};
-
-/*
- * LDS (bytes): per plane, S rows [32][64 B] and Q rows [32 u][33 w][64 B]
- * (one row = one kappa chunk of one (u, w) pair: 8 FP64, 16 FP32 or 32
- * halves).  Q rows are skewed (33 w per u) so 16 consecutive u at one w
- * land in different banks, and the four 16-byte chunks of each row are
- * XOR-swizzled by (row / 4) % 4.  Split forms keep hi and lo halves in two
- * planes.
- */
-template <int F> struct Lds {
-	typedef Form<F> FM;
-	static constexpr int QP = 33;
-	static constexpr int ROWB = 64;
-	static constexpr int S_PB = 32 * ROWB;        /* S plane bytes */
-	static constexpr int Q_PB = 32 * QP * ROWB;   /* Q plane bytes */
-	static constexpr int STAGE_BYTES = FM::NP * (S_PB + Q_PB);
-	/* FP64/FP32 double-buffer their stages; split forms use one */
-	static constexpr int NSTAGE = FM::SPLIT ? 1 : 2;
-	static constexpr int X_BYTES = 4 * 32 * 64 * (int)sizeof(typename FM::AE);
-	static constexpr int BYTES = NSTAGE * STAGE_BYTES > X_BYTES ?
-	    NSTAGE * STAGE_BYTES : X_BYTES;
-	/* byte offset of 16-byte chunk `ch` of row `row` */
-	static __device__ __forceinline__ int off(int row, int ch)
-	{
-		return row * ROWB + ((ch ^ ((row >> 2) & 3)) << 4);
-	}
-	/* byte offset of element k (of size ES) of row `row` */
-	template <int ES>
-	static __device__ __forceinline__ int eoff(int row, int k)
-	{
-		return off(row, (k * ES) >> 4) + ((k * ES) & 15);
-	}
-};
 
 /* ---------------------------------------------------------------- MFMA */
 
@@ -130,18 +98,34 @@ mma(float a, float b, v4f c)
 	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+/* one K=16 bf16 / f16 MFMA on operands given as two 32-bit words each
+ * (halves k0, k1 and k2, k3 of the lane's kappa) */
 template <int F>
 __device__ __forceinline__ v4f
-mma8(v8u a, v8u b, v4f c)
+mma16(uint32_t a01, uint32_t a23, uint32_t b01, uint32_t b23, v4f c)
 {
+	const v2u a = {a01, a23}, b = {b01, b23};
 	if constexpr (F == PT_FORM_BF16X3)
-		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-		    __builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c,
-		    0, 0, 0);
+		return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+		    __builtin_bit_cast(v4s, a), __builtin_bit_cast(v4s, b), c, 0, 0,
+		    0);
 	else
-		return __builtin_amdgcn_mfma_f32_16x16x32_f16(
-		    __builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c,
-		    0, 0, 0);
+		return __builtin_amdgcn_mfma_f32_16x16x16f16(
+		    __builtin_bit_cast(v4h, a), __builtin_bit_cast(v4h, b), c, 0, 0,
+		    0);
+}
+
+/* hi*hi + hi*lo + lo*hi (FP32 accumulation) of one split operand pair.
+ * Operand words (LDS layout of the split forms): w0 = hi k0 k1,
+ * w1 = lo k0 k1, w2 = hi k2 k3, w3 = lo k2 k3; the small products first. */
+template <int F>
+__device__ __forceinline__ v4f
+mma_split(v4u a, v4u b, v4f c)
+{
+	c = mma16<F>(a[1], a[3], b[0], b[2], c);
+	c = mma16<F>(a[0], a[2], b[1], b[3], c);
+	c = mma16<F>(a[0], a[2], b[0], b[2], c);
+	return c;
 }
 
 template <class G>
@@ -320,101 +304,6 @@ brsrc(const void *base, int64_t bytes)
 
 #define PT_OOR 0x80000000u  /* an offset every descriptor rejects */
 
-typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-
-/*
- * One staging step = (term, kappa chunk).  S: SEPT consecutive kappa of one
- * row per thread, read in the API layout.  Q: 16-byte pieces of the packed
- * copy; thread tid always moves chunk (tid % NV) of pairs
- * tid / NV + n * (256 / NV), so one wave load covers 16 (or 8) whole pair
- * groups of consecutive memory.
- */
-template <int F, int NPC, int SEPT>
-__device__ __forceinline__ void
-load_step(const Step &s, int kc, int tid, v4u (&qr)[NPC],
-    typename Form<F>::G (&sr)[SEPT])
-{
-	typedef typename Form<F>::G G;
-	constexpr int KC = Form<F>::KC;
-	constexpr int ES = (int)sizeof(G);
-	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
-	constexpr int NV = GB / 16, LNV = NV == 8 ? 3 : 2;
-	{
-		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
-		const int sg = s.s0 + row;
-		const bool rv = row < s.sn && sg < s.sext;
-		const __amdgpu_buffer_rsrc_t rs = brsrc(s.sb0, s.sbytes);
-#pragma unroll
-		for (int e = 0; e < SEPT; ++e) {
-			const int kg = kc + k0 + e;
-			const uint32_t o = (rv && kg < s.K) ?
-			    (uint32_t)(((int64_t)sg * s.s_st + kg) * ES) : PT_OOR;
-			sr[e] = neg_if(bload<G>(rs, o), s.neg);
-		}
-	}
-	/* packed Q: descriptor over the kappa group of this chunk */
-	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)(kc / KC) * s.qgs,
-	    s.qgs);
-	const int ch = tid & (NV - 1);
-	const int npc = ((32 * s.qwe) << LNV) >> 8;
-#pragma unroll
-	for (int n = 0; n < NPC; ++n) {
-		const int pair = (tid >> LNV) + n * (256 >> LNV);
-		int u, w;
-		pair_uw(s, pair, u, w);
-		const int ug = s.u0 + u, wg = s.w0 + w;
-		const bool v = n < npc && ug < s.uext && wg < s.wext &&
-		    (!s.packed || ug > wg);
-		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
-		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
-		const uint32_t o = v ? (uint32_t)(pidx * GB + ch * 16) : PT_OOR;
-		qr[n] = __builtin_bit_cast(v4u,
-		    __builtin_amdgcn_raw_buffer_load_b128(rq, (int)o, 0, 0));
-	}
-}
-
-template <int F, int NPC, int SEPT>
-__device__ __forceinline__ void
-store_step(const Step &s, int tid, char *sS, char *sQ, const v4u (&qr)[NPC],
-    const typename Form<F>::G (&sr)[SEPT])
-{
-	typedef Form<F> FM;
-	typedef Lds<F> LD;
-	constexpr int KC = FM::KC;
-	constexpr int GB = F >= PT_FORM_BF16X3 ? 128 : 64;
-	constexpr int NV = GB / 16, LNV = NV == 8 ? 3 : 2;
-	{
-		const int row = tid / (KC / SEPT), k0 = (tid % (KC / SEPT)) * SEPT;
-		if constexpr (FM::SPLIT) {
-			/* SEPT = 4 kappa: 8 bytes of hi halves, 8 of lo halves */
-			uint32_t h01 = (sr[0] & 0xffffu) | (sr[1] << 16);
-			uint32_t h23 = (sr[2] & 0xffffu) | (sr[3] << 16);
-			uint32_t l01 = (sr[0] >> 16) | (sr[1] & 0xffff0000u);
-			uint32_t l23 = (sr[2] >> 16) | (sr[3] & 0xffff0000u);
-			const int o = LD::template eoff<2>(row, k0);
-			*(uint2 *)(sS + o) = make_uint2(h01, h23);
-			*(uint2 *)(sS + LD::S_PB + o) = make_uint2(l01, l23);
-		} else {
-			typedef typename FM::L L;
-#pragma unroll
-			for (int e = 0; e < SEPT; ++e)
-				*(L *)(sS + LD::template eoff<(int)sizeof(L)>(row,
-				    k0 + e)) = sr[e];
-		}
-	}
-	const int ch = tid & (NV - 1);
-	const int npc = ((32 * s.qwe) << LNV) >> 8;
-#pragma unroll
-	for (int n = 0; n < NPC; ++n) {
-		const int pair = (tid >> LNV) + n * (256 >> LNV);
-		int u, w;
-		pair_uw(s, pair, u, w);
-		const int row = u * LD::QP + w;
-		if (n < npc)
-			*(v4u *)(sQ + (ch >> 2) * LD::Q_PB + LD::off(row, ch & 3)) =
-			    qr[n];
-	}
-}
 
 /*
  * FP64 / FP32 staging: NSLOT slots, each holding one step (term, kappa
@@ -529,10 +418,33 @@ __device__ __forceinline__ void
 store_s(char *sS, int lofs, const typename Form<F>::G (&sr)[SEPT])
 {
 	typedef typename Form<F>::G G;
-	if (lofs >= 0) {
+	if (lofs < 0)
+		return;
+	if constexpr (Form<F>::SPLIT) {
+		/* two kappa (hi | lo << 16 each) -> hi pair word, lo pair word */
+		static_assert(SEPT == 2, "split S pairs");
+		const uint32_t h = (sr[0] & 0xffffu) | (sr[1] << 16);
+		const uint32_t l = (sr[0] >> 16) | (sr[1] & 0xffff0000u);
+		*(v2u *)(sS + lofs) = v2u{h, l};
+	} else {
 #pragma unroll
 		for (int e = 0; e < SEPT; ++e)
 			*(G *)(sS + lofs + e * (int)sizeof(G)) = sr[e];
+	}
+}
+
+/* a split-form operand from LDS: 16 bytes (type A: hi k0 k1, lo k0 k1,
+ * hi k2 k3, lo k2 k3) or 8 bytes (type B: hi k0 k1, lo k0 k1, with k2 and
+ * k3 zero in both operands) */
+template <bool TA>
+__device__ __forceinline__ v4u
+split_rd(const char *p)
+{
+	if constexpr (TA)
+		return *(const v4u *)p;
+	else {
+		const v2u t = *(const v2u *)p;
+		return v4u{t[0], t[1], 0u, 0u};
 	}
 }
 
@@ -556,17 +468,29 @@ mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 {
 	typedef typename Form<F>::L L;
 	constexpr int NB = TA ? 16 : 8, WB = TA ? 64 : 32;
-	typedef typename Chunk<L, NB>::V V;
-	constexpr int NE = Chunk<L, NB>::N;
 	const int ws = RA ? wa : wb, wu = RA ? wb : wa;
-	const V f = *(const V *)(sS + (16 * ws + ln) * Stg::S_ROW + NB * lg);
+	const char *pf = sS + (16 * ws + ln) * Stg::S_ROW + NB * lg;
 	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + NB * lg;
+	if constexpr (Form<F>::SPLIT) {
+		const v4u f = split_rd<TA>(pf);
 #pragma unroll
-	for (int x = 0; x < TCW; ++x) {
-		const V q = *(const V *)(pv + WB * x);
+		for (int x = 0; x < TCW; ++x) {
+			const v4u q = split_rd<TA>(pv + WB * x);
+			ac[x] = RA ? mma_split<F>(f, q, ac[x]) :
+			    mma_split<F>(q, f, ac[x]);
+		}
+	} else {
+		typedef typename Chunk<L, NB>::V V;
+		constexpr int NE = Chunk<L, NB>::N;
+		const V f = *(const V *)pf;
 #pragma unroll
-		for (int e = 0; e < NE; ++e)
-			ac[x] = RA ? mma(f[e], q[e], ac[x]) : mma(q[e], f[e], ac[x]);
+		for (int x = 0; x < TCW; ++x) {
+			const V q = *(const V *)(pv + WB * x);
+#pragma unroll
+			for (int e = 0; e < NE; ++e)
+				ac[x] = RA ? mma(f[e], q[e], ac[x]) :
+				    mma(q[e], f[e], ac[x]);
+		}
 	}
 }
 
@@ -580,82 +504,37 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
     int wa, int wb, int ln, int lg)
 {
 	typedef typename Form<F>::L L;
-	typedef typename Chunk<L, 8>::V V;
-	constexpr int NE = Chunk<L, 8>::N, NCB = TCW / 16;
-	V f[NCB];
-#pragma unroll
-	for (int cb = 0; cb < NCB; ++cb)
-		f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
+	constexpr int NCB = TCW / 16;
 	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR + 16 * wb * 32 + 8 * lg;
-#pragma unroll
-	for (int b = 0; b < 16; ++b) {
-		const V q = *(const V *)(pv + 32 * b);
+	if constexpr (Form<F>::SPLIT) {
+		v4u f[NCB];
 #pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
+			f[cb] = split_rd<false>(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
 #pragma unroll
-			for (int e = 0; e < NE; ++e)
-				ac[16 * cb + b] = mma(q[e], f[cb][e], ac[16 * cb + b]);
-	}
-}
-
-/* split forms: MFMA over one staged chunk into a fixed accumulator set */
-template <int F, int NCB>
-__device__ __forceinline__ void
-mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
-    const char *sQ, int wa, int wb, int ln, int lg)
-{
-	typedef Form<F> FM;
-	typedef Lds<F> LD;
-	constexpr int TCW = 16 * NCB, QP = LD::QP;
-	{
-#define LD8(base, plane, row) \
-	(*(const v8u *)((base) + (plane) + LD::off((row), lg)))
-		if (role == PT_ROLE_A) {
-			const int rs = 16 * wa + ln;
-			const v8u ah = LD8(sS, 0, rs), al = LD8(sS, LD::S_PB, rs);
-			const int r0 = (16 * wb + ln) * QP;
+		for (int b = 0; b < 16; ++b) {
+			const v4u q = split_rd<false>(pv + 32 * b);
 #pragma unroll
-			for (int x = 0; x < TCW; ++x) {
-				const v8u bh = LD8(sQ, 0, r0 + x);
-				const v8u bl = LD8(sQ, LD::Q_PB, r0 + x);
-				ac[x] = mma8<F>(al, bh, ac[x]);
-				ac[x] = mma8<F>(ah, bl, ac[x]);
-				ac[x] = mma8<F>(ah, bh, ac[x]);
-			}
-		} else if (role == PT_ROLE_B) {
-			const int rs = 16 * wb + ln;
-			const v8u bh = LD8(sS, 0, rs), bl = LD8(sS, LD::S_PB, rs);
-			const int r0 = (16 * wa + ln) * QP;
-#pragma unroll
-			for (int x = 0; x < TCW; ++x) {
-				const v8u ah = LD8(sQ, 0, r0 + x);
-				const v8u al = LD8(sQ, LD::Q_PB, r0 + x);
-				ac[x] = mma8<F>(al, bh, ac[x]);
-				ac[x] = mma8<F>(ah, bl, ac[x]);
-				ac[x] = mma8<F>(ah, bh, ac[x]);
-			}
-		} else {
-			v8u bh[NCB], bl[NCB];
-#pragma unroll
-			for (int cb = 0; cb < NCB; ++cb) {
-				bh[cb] = LD8(sS, 0, 16 * cb + ln);
-				bl[cb] = LD8(sS, LD::S_PB, 16 * cb + ln);
-			}
-			const int r0 = (16 * wa + ln) * QP + 16 * wb;
-#pragma unroll
-			for (int b = 0; b < 16; ++b) {
-				const v8u ah = LD8(sQ, 0, r0 + b);
-				const v8u al = LD8(sQ, LD::Q_PB, r0 + b);
-#pragma unroll
-				for (int cb = 0; cb < NCB; ++cb) {
-					const int x = 16 * cb + b;
-					ac[x] = mma8<F>(al, bh[cb], ac[x]);
-					ac[x] = mma8<F>(ah, bl[cb], ac[x]);
-					ac[x] = mma8<F>(ah, bh[cb], ac[x]);
-				}
-			}
+			for (int cb = 0; cb < NCB; ++cb)
+				ac[16 * cb + b] = mma_split<F>(q, f[cb], ac[16 * cb + b]);
 		}
-#undef LD8
+	} else {
+		typedef typename Chunk<L, 8>::V V;
+		constexpr int NE = Chunk<L, 8>::N;
+		V f[NCB];
+#pragma unroll
+		for (int cb = 0; cb < NCB; ++cb)
+			f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
+#pragma unroll
+		for (int b = 0; b < 16; ++b) {
+			const V q = *(const V *)(pv + 32 * b);
+#pragma unroll
+			for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+				for (int e = 0; e < NE; ++e)
+					ac[16 * cb + b] = mma(q[e], f[cb][e],
+					    ac[16 * cb + b]);
+		}
 	}
 }
 
@@ -665,8 +544,7 @@ mma_step(typename Form<F>::ACC (&ac)[16 * NCB], int role, const char *sS,
  * workgroup's LDS (two slots, 76 KB) and registers (<= 256 per lane) allow,
  * so that one workgroup's staging and barriers overlap the other's MFMAs */
 template <int F, int NSET> struct Occ {
-	static constexpr int WG_PER_CU =
-	    (F == PT_FORM_FP64 || F == PT_FORM_FP32) && NSET == 1 ? 2 : 1;
+	static constexpr int WG_PER_CU = F == PT_FORM_FP64 && NSET == 2 ? 1 : 2;
 };
 
 /*
@@ -680,7 +558,6 @@ __device__ __forceinline__ void
 tile_item(const pt_launch &p, int64_t it)
 {
 	typedef Form<F> FM;
-	typedef Lds<F> LD;
 	typedef typename FM::G G;
 	typedef typename FM::L L;
 	typedef typename FM::AE AE;
@@ -689,22 +566,16 @@ tile_item(const pt_launch &p, int64_t it)
 	constexpr int KC = FM::KC;
 	constexpr int NCB = (F == PT_FORM_FP32 && NSET == 1) ? 2 : 1; /* pt_tcw */
 	constexpr int TCW = 16 * NCB;
-	/* packed group bytes: 128 (split), 32 (FP64 / FP32) */
-	constexpr int GB = FM::SPLIT ? 128 : 32;
-	constexpr int NPC = FM::SPLIT ? 32 * 32 * (GB / 16) / PT_THREADS : 1;
-	/* S elements per thread (split: per 32-kappa chunk; FP64 / FP32: per
-	 * step of either type) */
-	constexpr int SEPT = FM::SPLIT ? 32 * KC / PT_THREADS :
-	    (F == PT_FORM_FP64 ? 1 : 2);
-	constexpr int SMEM = FM::SPLIT ? LD::BYTES : Stg::BYTES;
+	constexpr int GB = 32;  /* bytes per packed kappa group and pair */
+	/* S elements per thread per step */
+	constexpr int SEPT = F == PT_FORM_FP64 ? 1 : 2;
+	constexpr int SMEM = Stg::BYTES;
 	static_assert(SMEM >= 4 * NSET * TCW * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
 
 	__shared__ __attribute__((aligned(16))) char smem[SMEM];
 	__shared__ double red[4];
 	__shared__ pt_term sterm[PT_MAX_TERMS];
-	char *sS = smem;
-	char *sQ = smem + FM::NP * LD::S_PB;
 	AE *sX = (AE *)smem;
 
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -772,7 +643,7 @@ tile_item(const pt_launch &p, int64_t it)
 		__syncthreads();
 	};
 
-	if constexpr (!FM::SPLIT) {
+	{
 		/*
 		 * Steps n = 0 .. N-1 walk the terms (role a, role b, role c) and
 		 * their kappa chunks; step n lives in slot n % NSLOT.  The loads of
@@ -950,38 +821,6 @@ tile_item(const pt_launch &p, int64_t it)
 		(void)t;
 		(void)kc;
 		(void)cur;
-	} else {
-		v4u qr[NPC];
-		G sr[SEPT];
-		load_step<F, NPC, SEPT>(cur, 0, tid, qr, sr);
-		while (t < nterm) {
-			if (t == p.nterm1 && kc == 0)
-				transpose(sX);
-			__syncthreads();
-			store_step<F, NPC, SEPT>(cur, tid, sS, sQ, qr, sr);
-			__syncthreads();
-			/* prefetch the next step while this one computes */
-			const int role = cur.role, set = cur.set;
-			int tn = t, kn = kc + KC;
-			if (kn >= cur.K) {
-				tn = t + 1;
-				kn = 0;
-			}
-			if (tn < nterm) {
-				if (tn != t)
-					cur = step_of(tn);
-				load_step<F, NPC, SEPT>(cur, kn, tid, qr, sr);
-			}
-			if (act) {
-				if (NSET == 1 || set == 0)
-					mma_step<F, NCB>(acc[0], role, sS, sQ, wa, wb, ln, lg);
-				else
-					mma_step<F, NCB>(acc[NSET - 1], role, sS, sQ, wa, wb,
-					    ln, lg);
-			}
-			t = tn;
-			kc = kn;
-		}
 	}
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
@@ -1180,6 +1019,7 @@ k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
 			for (int kk = 0; kk < KC; ++kk)
 				dp[kk] = k0 + kk < K ? sp[(int64_t)kk * PL] : 0.0f;
 		} else {
+			/* per kappa pair q: hi 2q, hi 2q + 1, lo 2q, lo 2q + 1 */
 			const double *sp = (const double *)src + sbase;
 			uint16_t *dp = (uint16_t *)dst + t * 2 * KC;
 			for (int kk = 0; kk < KC; ++kk) {
@@ -1187,8 +1027,8 @@ k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
 				if (k0 + kk < K)
 					split2(sp[(int64_t)kk * PL] * scale,
 					    mode == PT_FORM_F16X3, hb, lb);
-				dp[kk] = hb;
-				dp[KC + kk] = lb;
+				dp[4 * (kk >> 1) + (kk & 1)] = hb;
+				dp[4 * (kk >> 1) + 2 + (kk & 1)] = lb;
 			}
 		}
 	}
