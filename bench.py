@@ -36,6 +36,39 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "f16x3": 2500.0}
 EXEC_FACTOR = {"fp64": 1, "fp32": 1, "bf16x3": 3, "f16x3": 3}
 # fastest published run of this system (README.md:52): 440 s on 4096 cores
 PUBLISHED_WALL_S = 440.0
+# HBM traffic of the dominant launch at the default configuration, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "k_tiles_fp64_rpt_o45_v477_traffic.json")
+
+
+def traffic_of(form, nbatch, o, v):
+    """HBM bytes per launch of the dominant kernel, if a counter summary
+    for this configuration is committed; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("form"), t.get("nbatch"), t.get("o"), t.get("v")) != \
+            (form, nbatch, o, v):
+        return None
+    return t.get("bytes_per_launch")
+
+
+def host_info():
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    info["cpu"] = ln.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            info["mem_gb"] = round(int(f.readline().split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return info
 
 
 def log(*a):
@@ -47,6 +80,13 @@ def run_form(api, form, dims, dev, args, nbatch):
     t0 = time.perf_counter()
     plan = api.Plan("rpt", form, dims, dev)
     prep = time.perf_counter() - t0
+    blk, tune_s = None, 0.0
+    if args.tune:
+        t0 = time.perf_counter()
+        blk = plan.tune(args.tune_sample)
+        tune_s = time.perf_counter() - t0
+        log(f"[bench] {form}: tile-order blocking {blk} "
+            f"(tuned in {tune_s:.1f} s)")
     ncase = plan.ncases
     nijk = [plan.nijk(sc) for sc in range(ncase)]
     fpi = [plan.flop_per_ijk(sc) for sc in range(ncase)]
@@ -90,7 +130,8 @@ def run_form(api, form, dims, dev, args, nbatch):
     dom = max(range(ncase), key=lambda sc: nijk[sc] * fpi[sc])
     dom_flop = sum((rng(dom, b)[1] - rng(dom, b)[0]) * fpi[dom]
                    for b in batches)
-    res = dict(form=form, prep_s=prep, wall_s=wall, flop=flop,
+    res = dict(form=form, prep_s=prep, tile_block=blk, tune_s=tune_s,
+               wall_s=wall, flop=flop,
                tflops=flop / wall / 1e12, ms_per_step=1e3 * wall / args.steps,
                kern_ms=kern_ms, dom_case=dom,
                dom_launch_ms=kern_ms[dom] / args.steps,
@@ -150,7 +191,9 @@ def cpu_baseline(api, o, v, seed):
     dt = time.perf_counter() - t0
     f = api.flop("rpt", (o, v))
     return {"value": round(f / dt / 1e12, 4), "unit": "TFLOP/s",
-            "cores": threads, "kind": kind,
+            "cores": threads, "kind": kind, "host": host_info(),
+            "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)" if kind ==
+            "reference" else "none (restatement)",
             "sample": f"libpt_rpt o={o} v={v} FP64 (full call, "
                       f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})"}
 
@@ -165,6 +208,10 @@ def main():
     ap.add_argument("--nbatch", type=int, default=128)
     ap.add_argument("--forms", default="fp64,bf16x3")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tune", type=int, default=1,
+                    help="autotune the tile order before timing (0: default)")
+    ap.add_argument("--tune-sample", type=int, default=8,
+                    help="ijk triples per spin case per tuning candidate")
     ap.add_argument("--batch", type=int, default=64,
                     help="systems in the batched-launch measurement (0: skip)")
     ap.add_argument("--cpu", default="16,160",
@@ -232,6 +279,7 @@ def main():
             "flop_per_step": r["flop"] / args.steps,
             "total_flop": r["total_flop"],
             "projected_full_T_wall_s": round(proj_wall, 1),
+            "tile_block": r["tile_block"],
         },
         "roofline": {
             "bound": "mfma",
@@ -240,7 +288,7 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": traffic_of(main_form, args.nbatch, args.o, args.v),
         },
     }
     for form in forms[1:]:
@@ -249,6 +297,7 @@ def main():
             "tflops": round(q["tflops"], 3),
             "ms_per_step": round(q["ms_per_step"], 2),
             "speedup_vs_" + main_form: round(q["tflops"] / r["tflops"], 3),
+            "tile_block": q["tile_block"],
             "projected_full_T_wall_s": round(q["total_flop"] /
                                              (q["tflops"] * 1e12), 1),
             "mfma_frac": round(q["dom_flop_per_launch"] * EXEC_FACTOR[form] /

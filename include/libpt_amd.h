@@ -80,6 +80,17 @@ int libpt_amd_plan_ijk_energies(libpt_amd_plan *plan, int sc, double *out);
 /* the ijk triples of spin case sc, 3 * n_ijk ints, in work-list order */
 int libpt_amd_plan_ijk_list(const libpt_amd_plan *plan, int sc, int32_t *out);
 
+/* Tile order of the work list: tiles (A, B, C) are visited in blocks of
+ * ga x gb x gc so that workgroups running together share operand slices
+ * in L2.  set_block applies one blocking; tune times candidate blockings
+ * on the first `sample_ijk` ijk triples of each spin case (0: 8), applies
+ * the fastest and records it for plans of the same (form, call, sizes),
+ * also in the file named by LIBPT_AMD_TUNE_CACHE when set; out3 (may be
+ * NULL) receives (ga, gb, gc).  LIBPT_AMD_TILE_BLOCK="ga,gb,gc" forces a
+ * blocking for new plans.  0 on success. */
+int libpt_amd_plan_set_block(libpt_amd_plan *plan, int ga, int gb, int gc);
+int libpt_amd_plan_tune(libpt_amd_plan *plan, int sample_ijk, int *out3);
+
 /* GPU-timed window on the plan's stream: begin/end bracket enqueued work */
 int libpt_amd_plan_timer_begin(libpt_amd_plan *plan);
 double libpt_amd_plan_timer_end_ms(libpt_amd_plan *plan);

@@ -95,6 +95,10 @@ def lib():
         L.libpt_amd_plan_ijk_energies.argtypes = [_p, _i, _p]
         L.libpt_amd_plan_ijk_list.restype = _i
         L.libpt_amd_plan_ijk_list.argtypes = [_p, _i, _p]
+        L.libpt_amd_plan_set_block.restype = _i
+        L.libpt_amd_plan_set_block.argtypes = [_p, _i, _i, _i]
+        L.libpt_amd_plan_tune.restype = _i
+        L.libpt_amd_plan_tune.argtypes = [_p, _i, _p]
         L.libpt_amd_plan_timer_begin.restype = _i
         L.libpt_amd_plan_timer_begin.argtypes = [_p]
         L.libpt_amd_plan_timer_end_ms.restype = _d
@@ -345,6 +349,18 @@ class Plan:
         if lib().libpt_amd_plan_ijk_list(self.h, sc, out.ctypes.data) != 0:
             raise LibptError(last_error())
         return out
+
+    def set_block(self, ga, gb, gc):
+        if lib().libpt_amd_plan_set_block(self.h, ga, gb, gc) != 0:
+            raise LibptError(last_error())
+
+    def tune(self, sample_ijk=8):
+        """Autotune the tile order (see libpt_amd_plan_tune); returns the
+        chosen (ga, gb, gc)."""
+        out = (C.c_int * 3)()
+        if lib().libpt_amd_plan_tune(self.h, sample_ijk, out) != 0:
+            raise LibptError(last_error())
+        return tuple(out)
 
     def timer_begin(self):
         lib().libpt_amd_plan_timer_begin(self.h)

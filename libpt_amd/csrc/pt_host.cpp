@@ -149,14 +149,13 @@ flop_mixed(double oa, double va, double ob, double vb)
 
 /* blocking of the tile order (A, B, C tiles per block) */
 struct TileBlock {
-	int ga = 2, gb = 2, gc = 8;
+	int ga = 8, gb = 8, gc = 2;
 };
 
-/* LIBPT_AMD_TILE_BLOCK="ga,gb,gc" overrides the default blocking */
-TileBlock
-tile_block_default()
+/* LIBPT_AMD_TILE_BLOCK="ga,gb,gc" forces a blocking */
+bool
+tile_block_env(TileBlock &tb)
 {
-	TileBlock tb;
 	const char *e = getenv("LIBPT_AMD_TILE_BLOCK");
 	int a, b, c;
 	if (e && sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && a > 0 && b > 0 &&
@@ -164,12 +163,86 @@ tile_block_default()
 		tb.ga = a;
 		tb.gb = b;
 		tb.gc = c;
+		return true;
 	}
-	return tb;
+	return false;
+}
+
+/*
+ * Tuning cache: the blocking chosen by libpt_amd_plan_tune per (form,
+ * call, oa, va, ob, vb), kept for the process and, when
+ * LIBPT_AMD_TUNE_CACHE names a file, read from and appended to it (one
+ * line per entry: form call oa va ob vb ga gb gc).
+ */
+struct TuneEntry {
+	int form, call;
+	int64_t oa, va, ob, vb;
+	TileBlock tb;
+};
+
+std::vector<TuneEntry> g_tune;
+bool g_tune_loaded = false;
+
+void
+tune_load()
+{
+	if (g_tune_loaded)
+		return;
+	g_tune_loaded = true;
+	const char *path = getenv("LIBPT_AMD_TUNE_CACHE");
+	FILE *f = path ? fopen(path, "r") : nullptr;
+	if (!f)
+		return;
+	TuneEntry t;
+	long long oa, va, ob, vb;
+	while (fscanf(f, "%d %d %lld %lld %lld %lld %d %d %d", &t.form, &t.call,
+	    &oa, &va, &ob, &vb, &t.tb.ga, &t.tb.gb, &t.tb.gc) == 9) {
+		t.oa = oa;
+		t.va = va;
+		t.ob = ob;
+		t.vb = vb;
+		if (t.tb.ga > 0 && t.tb.gb > 0 && t.tb.gc > 0)
+			g_tune.push_back(t);
+	}
+	fclose(f);
+}
+
+bool
+tune_lookup(int form, int call, int64_t oa, int64_t va, int64_t ob,
+    int64_t vb, TileBlock &tb)
+{
+	tune_load();
+	/* the latest entry wins */
+	for (size_t n = g_tune.size(); n-- > 0;) {
+		const TuneEntry &t = g_tune[n];
+		if (t.form == form && t.call == call && t.oa == oa && t.va == va &&
+		    t.ob == ob && t.vb == vb) {
+			tb = t.tb;
+			return true;
+		}
+	}
+	return false;
+}
+
+void
+tune_store(int form, int call, int64_t oa, int64_t va, int64_t ob,
+    int64_t vb, const TileBlock &tb)
+{
+	tune_load();
+	g_tune.push_back({form, call, oa, va, ob, vb, tb});
+	const char *path = getenv("LIBPT_AMD_TUNE_CACHE");
+	FILE *f = path ? fopen(path, "a") : nullptr;
+	if (f) {
+		fprintf(f, "%d %d %lld %lld %lld %lld %d %d %d\n", form, call,
+		    (long long)oa, (long long)va, (long long)ob, (long long)vb,
+		    tb.ga, tb.gb, tb.gc);
+		fclose(f);
+	}
 }
 
 struct Case {
-	TileBlock block = tile_block_default();
+	TileBlock block;
+	int same = 0, nA = 0, nC = 0, tcw = 16;  /* tile-list geometry */
 	pt_launch L;
 	std::vector<pt_term> terms;
 	pt_term *d_terms = nullptr;
@@ -238,6 +311,52 @@ dev_alloc(void **p, size_t bytes)
 	return true;
 }
 
+/* Tiles in blocks of GA x GB x GC (A, B, C): the workgroups that run at the
+ * same time on one XCD take consecutive items, and a compact block lets
+ * them share the Q slices of all three roles in L2 (Q[B][C] across A,
+ * Q[A][C] across B, Q[A][B] across C).  Writes 3 ints per tile; returns
+ * the tile count (the same for every blocking). */
+int
+tile_order(const Case &c, const TileBlock &tb, int32_t *out)
+{
+	const int nA = c.nA, nC = c.nC, tcw = c.tcw;
+	int nt = 0;
+	for (int A0 = 0; A0 < nA; A0 += tb.ga)
+	for (int B0 = 0; B0 < A0 + tb.ga && B0 < nA; B0 += tb.gb)
+	for (int C0 = 0; C0 < nC; C0 += tb.gc)
+		for (int A = A0; A < A0 + tb.ga && A < nA; ++A)
+		for (int B = B0; B < B0 + tb.gb && B <= A; ++B)
+		for (int Cb = C0; Cb < C0 + tb.gc && Cb < nC; ++Cb) {
+			/* same spin needs some b > c: c_min < b_max */
+			if (c.same && Cb * tcw >= B * PT_TB + PT_TB - 1)
+				continue;
+			out[3 * nt] = A;
+			out[3 * nt + 1] = B;
+			out[3 * nt + 2] = Cb;
+			++nt;
+		}
+	return nt;
+}
+
+/* a new blocking for an existing case (same tiles, new order) */
+bool
+case_set_block(Case &c, const TileBlock &tb)
+{
+	if (c.n_tiles == 0)
+		return true;
+	HostBuf<int32_t> tiles;
+	if (!tiles.alloc(3 * (size_t)c.n_tiles))
+		return false;
+	if (tile_order(c, tb, tiles.p) != c.n_tiles) {
+		set_err("tile order changed the tile count");
+		return false;
+	}
+	HCHK(hipMemcpy(c.d_tiles, tiles.p, 3 * sizeof(int32_t) * (size_t)c.n_tiles,
+	    hipMemcpyHostToDevice), false);
+	c.block = tb;
+	return true;
+}
+
 /* work list of one spin case: ijk items x (A, B, C) tiles.  A >= B; for
  * the same-spin case (a > b > c) only c tiles that reach below the top of
  * the B block. */
@@ -258,31 +377,14 @@ case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na, int64_t nc,
 				ijk.p[3 * it + 2] = (int32_t)k;
 				++it;
 			}
-	int nA = (int)((na + PT_TA - 1) / PT_TA);
-	int nC = (int)((nc + tcw - 1) / tcw);
+	c.same = same;
+	c.nA = (int)((na + PT_TA - 1) / PT_TA);
+	c.nC = (int)((nc + tcw - 1) / tcw);
+	c.tcw = tcw;
 	HostBuf<int32_t> tiles;
-	if (!tiles.alloc(3 * (size_t)nA * (nA + 1) / 2 * nC))
+	if (!tiles.alloc(3 * (size_t)c.nA * (c.nA + 1) / 2 * c.nC))
 		return false;
-	/* Tiles in blocks of GA x GB x GC (A, B, C): the workgroups that run
-	 * at the same time on one XCD take consecutive items, and a compact
-	 * block lets them share the Q slices of all three roles in L2
-	 * (Q[B][C] across A, Q[A][C] across B, Q[A][B] across C). */
-	const TileBlock tb = c.block;
-	int nt = 0;
-	for (int A0 = 0; A0 < nA; A0 += tb.ga)
-	for (int B0 = 0; B0 < A0 + tb.ga && B0 < nA; B0 += tb.gb)
-	for (int C0 = 0; C0 < nC; C0 += tb.gc)
-		for (int A = A0; A < A0 + tb.ga && A < nA; ++A)
-		for (int B = B0; B < B0 + tb.gb && B <= A; ++B)
-		for (int Cb = C0; Cb < C0 + tb.gc && Cb < nC; ++Cb) {
-			/* same spin needs some b > c: c_min < b_max */
-			if (same && Cb * tcw >= B * PT_TB + PT_TB - 1)
-				continue;
-			tiles.p[3 * nt] = A;
-			tiles.p[3 * nt + 1] = B;
-			tiles.p[3 * nt + 2] = Cb;
-			++nt;
-		}
+	const int nt = tile_order(c, c.block, tiles.p);
 	c.n_ijk = nijk;
 	c.n_tiles = nt;
 	if (!dev_alloc((void **)&c.d_ijk, 3 * sizeof(int32_t) * (size_t)nijk) ||
@@ -575,8 +677,14 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	if (!restricted)
 		specs.push_back({PT_AAB, 1});
 
+	/* tile-order blocking: forced by the environment, else the tuning
+	 * cache, else the default */
+	TileBlock blk;
+	if (!tile_block_env(blk))
+		(void)tune_lookup(form, call, oa, va, ob, vb, blk);
 	for (const CaseSpec &cs : specs) {
 		Case c;
+		c.block = blk;
 		memset(&c.L, 0, sizeof(c.L));
 		const int sp = cs.spin;
 		/* block indices of the pair spin (same) and mixed blocks */
@@ -791,6 +899,86 @@ libpt_amd_plan_timer_end_ms(libpt_amd_plan *pl)
 	HCHK(hipEventSynchronize(pl->ev1), NAN);
 	HCHK(hipEventElapsedTime(&ms, pl->ev0, pl->ev1), NAN);
 	return (double)ms;
+}
+
+extern "C" int
+libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
+{
+	if (!pl || ga <= 0 || gb <= 0 || gc <= 0) {
+		set_err("bad plan or blocking");
+		return -1;
+	}
+	TileBlock tb;
+	tb.ga = ga;
+	tb.gb = gb;
+	tb.gc = gc;
+	HCHK(hipStreamSynchronize(pl->stream), -1);
+	for (Case &c : pl->cases)
+		if (!case_set_block(c, tb))
+			return -1;
+	return 0;
+}
+
+/*
+ * Autotuner of the tile order: for each candidate blocking, time the first
+ * `sample` ijk triples of every spin case (after one untimed pass), weight
+ * each case by its ijk count, keep the fastest, and record it in the
+ * tuning cache.  The energies of the sampled items are overwritten by the
+ * next run of the plan.
+ */
+extern "C" int
+libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out3)
+{
+	if (!pl) {
+		set_err("null plan");
+		return -1;
+	}
+	static const int cand[][3] = {
+		{1, 1, 64}, {2, 2, 8}, {4, 4, 2}, {4, 4, 4}, {8, 4, 2},
+		{8, 8, 1}, {8, 8, 2}, {16, 16, 1}, {4, 2, 8}, {2, 2, 16},
+	};
+	const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+	if (sample <= 0)
+		sample = 8;
+	double best = 0;
+	int bi = -1;
+	for (int ci = -1; ci < ncand; ++ci) {
+		/* ci = -1: one untimed warm pass with the first candidate */
+		const int *cb = cand[ci < 0 ? 0 : ci];
+		if (libpt_amd_plan_set_block(pl, cb[0], cb[1], cb[2]) != 0)
+			return -1;
+		double t = 0;
+		for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
+			const Case &c = pl->cases[sc];
+			if (c.n_ijk == 0)
+				continue;
+			const int64_t m = c.n_ijk < sample ? c.n_ijk : sample;
+			if (libpt_amd_plan_timer_begin(pl) != 0 ||
+			    libpt_amd_plan_run(pl, sc, 0, m) != 0)
+				return -1;
+			const double ms = libpt_amd_plan_timer_end_ms(pl);
+			if (isnan(ms))
+				return -1;
+			t += ms * (double)c.n_ijk / (double)m;
+		}
+		if (ci >= 0 && (bi < 0 || t < best)) {
+			best = t;
+			bi = ci;
+		}
+	}
+	if (libpt_amd_plan_set_block(pl, cand[bi][0], cand[bi][1], cand[bi][2]))
+		return -1;
+	TileBlock tb;
+	tb.ga = cand[bi][0];
+	tb.gb = cand[bi][1];
+	tb.gc = cand[bi][2];
+	tune_store(pl->form, pl->call, pl->oa, pl->va, pl->ob, pl->vb, tb);
+	if (out3) {
+		out3[0] = tb.ga;
+		out3[1] = tb.gb;
+		out3[2] = tb.gc;
+	}
+	return 0;
 }
 
 /* ---------------------------------------------------------------------
