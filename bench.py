@@ -173,6 +173,56 @@ def run_batch(api, form, n, seed):
             "E_sum": float(np.sum(e))}
 
 
+# the other configurations of the metric's definition, at their real shapes:
+# (name, call, dims, fraction of each spin case's ijk list, generator args)
+CONFIGS = [
+    ("rpt o=8 v=64", "rpt", (8, 64), 1, dict(sig_f=1e-2)),
+    ("upt 24/200/20/204", "upt", (24, 200, 20, 204), 1, dict()),
+    ("rft o=30 v=300 omega=0.30", "rft", (30, 300), 16, dict(omega=0.3)),
+    ("uft 16/160/14/160 omega=0.30", "uft", (16, 160, 14, 160), 1,
+     dict(omega=0.3)),
+]
+
+
+def run_config(api, name, kind, dims, frac, kw, forms, seed):
+    """One configuration in each form: the first 1/frac of every spin
+    case's ijk list (all of it for frac = 1), GPU-timed, with the energy of
+    the part run and its difference from the first form."""
+    spec = api.make_spec(kind, dims, seed=seed, **kw)
+    dev = api.synth_device(kind, spec)
+    out = {"config": name, "call": kind, "dims": list(dims),
+           "fraction_of_ijk": f"1/{frac}"}
+    try:
+        e0 = None
+        for form in forms:
+            t0 = time.perf_counter()
+            with api.Plan(kind, form, dims, dev) as pl:
+                prep = time.perf_counter() - t0
+                ms, flop, e = 0.0, 0.0, 0.0
+                for sc in range(pl.ncases):
+                    n = pl.nijk(sc)
+                    m = n if frac == 1 else max(1, n // frac)
+                    if n == 0:
+                        continue
+                    pl.timer_begin()
+                    pl.run(sc, 0, m)
+                    ms += pl.timer_end_ms()
+                    flop += m * pl.flop_per_ijk(sc)
+                    e += pl.factor(sc) * float(pl.ijk_energies(sc)[:m].sum())
+            r = {"tflops": round(flop / ms / 1e9, 3), "kernel_s": round(
+                ms / 1e3, 3), "prep_s": round(prep, 3), "flop": flop,
+                 "E": e}
+            if e0 is None:
+                e0 = e
+            else:
+                r["rel_dE_vs_" + forms[0]] = abs(e - e0) / abs(e0)
+            out[form] = r
+            log(f"[bench] {name} {form}: {r}")
+    finally:
+        dev.close()
+    return out
+
+
 def cpu_baseline(api, o, v, seed):
     """The reference libpt compiled from its sources (OpenMP, MKL in
     sequential mode) on a bounded sample; falls back to the oracle."""
@@ -216,6 +266,8 @@ def main():
                          "profiled run equal to the timed launches")
     ap.add_argument("--tune-sample", type=int, default=8,
                     help="ijk triples per spin case per tuning candidate")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also run the other configurations (0: skip)")
     ap.add_argument("--batch", type=int, default=64,
                     help="systems in the batched-launch measurement (0: skip)")
     ap.add_argument("--cpu", default="16,160",
@@ -313,6 +365,9 @@ def main():
             "rel_dE": abs(q["e_batches"] - r["e_batches"]) /
                       abs(r["e_batches"]),
         }
+    if args.configs:
+        out["configs"] = [run_config(api, *c, forms, args.seed)
+                          for c in CONFIGS]
     if args.batch > 0:
         out["batch"] = {}
         for form in forms:

@@ -102,26 +102,41 @@ struct pt_epi {
 
 struct pt_launch {
 	const void *buf[PT_MAX_BUFS];
-	const struct pt_term *terms; /* device array: role a, role b, role c */
-	int32_t nterm1, nterm2;
+	const struct pt_term *terms; /* device array: per W set, role a, role b,
+	                                role c */
+	int32_t nterm1, nterm2;      /* per W set: roles a and b, role c */
 	int32_t kind, nset;
 	int64_t oa, va, ob, vb;      /* aaa: ob = oa, vb = va */
 	int32_t na, nc;              /* extents of a (= b) and c */
 	int32_t tcw;                 /* c tile width (pt_tcw) */
-	int32_t nterm_a;             /* phase-1 terms with role a (they come
-	                                first, then role b) */
+	int32_t nterm_a;             /* per W set: terms with role a */
 	const int32_t *ijk;          /* [n_ijk][3] */
 	const int32_t *tiles;        /* [n_tiles][3]: blocks (A, B, C), A >= B */
 	int32_t n_tiles, pad_;
 	int64_t item0, n_chunk;      /* items [item0, item0 + n_chunk) */
 	double *partial;             /* energy of each item */
 	double inv_scale[2];         /* per W set: 1 / (operand scale product) */
+	void *wsave;                 /* two W sets: W of set 0 per workgroup
+	                                (pt_wsave_bytes each), else unused */
+	int32_t nslot, pad2_;        /* workgroups a launch may use (wsave
+	                                slots) */
 	struct pt_epi epi;
 };
 
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* persistent workgroups per launch: two per CU */
+#define PT_WG_PER_CU 2
+
+/* bytes of one workgroup's saved W set: 4 waves x TCW blocks x 4 values x
+ * 64 lanes */
+static inline int64_t
+pt_wsave_bytes(int form, int nset)
+{
+	return (int64_t)4 * pt_tcw(form, nset) * 4 * 64 * (form == 0 ? 8 : 4);
+}
+
 /* kernel launchers (pt_kernel.hip); return hipError_t as int */
 int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
 /* the kernels built without VGPR-form MFMA (pt_kernel_agpr.hip): FP64
@@ -129,11 +144,13 @@ int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
 int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
 /* batched launch: device array of nseg launches (partial, ijk, tiles,
  * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
- * every item of every segment in one grid (several above 2^22 items) */
+ * every item of every segment in one grid (several above 2^22 items).
+ * Two W sets: a persistent grid of at most nslot workgroups, workgroup g
+ * using save area g of its segment's wsave */
 int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *stream);
+    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream);
 int pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *stream);
+    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,
     const int64_t *begin, int nseg, double *out, void *stream);

@@ -61,6 +61,8 @@ now_s()
 		}                                                            \
 	} while (0)
 
+int g_nslot = 8;
+
 bool
 dev_init()
 {
@@ -78,6 +80,12 @@ dev_init()
 		return false;
 	}
 	HCHK(hipSetDevice(dev), false);
+	int cus = 0;
+	HCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+	    dev), false);
+	/* persistent workgroups per launch: a multiple of 8 (one share per
+	 * XCD) */
+	g_nslot = ((cus > 0 ? cus : 8) * PT_WG_PER_CU + 7) / 8 * 8;
 	g_dev_ready = true;
 	return true;
 }
@@ -431,7 +439,7 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 			t.q_off[x] = t.q_off[x] / slab * pslab;
 	}
 	L.nterm1 = pt_terms_order(c.terms, &L.nterm_a);
-	L.nterm2 = (int32_t)c.terms.size() - L.nterm1;
+	L.nterm2 = (int32_t)c.terms.size() / nset - L.nterm1;
 	if (!dev_alloc((void **)&c.d_terms, sizeof(pt_term) * c.terms.size()))
 		return false;
 	HCHK(hipMemcpy(c.d_terms, c.terms.data(),
@@ -504,6 +512,13 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	PFAIL(hipEventCreate(&pl->ev0) == hipSuccess &&
 	    hipEventCreate(&pl->ev1) == hipSuccess);
 	PFAIL(dev_alloc((void **)&pl->d_out, 4 * sizeof(double)));
+	/* two W sets: W of set 0 is saved per workgroup between the passes */
+	void *wsave = nullptr;
+	if (ft) {
+		PFAIL(dev_alloc(&wsave, (size_t)(g_nslot *
+		    pt_wsave_bytes(form, 2))));
+		pl->owned.push_back(wsave);
+	}
 
 	int64_t n[10];
 	const int narr = api_sizes(call, oa, va, ob, vb, n);
@@ -709,6 +724,8 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 			c.L.buf[sl + 11] = qT[s].p[mixC];
 			c.L.inv_scale[s] = 1.0 / (pl->scale[2 * s] * pl->scale[2 * s + 1]);
 		}
+		c.L.wsave = wsave;
+		c.L.nslot = g_nslot;
 		pt_epi &e = c.L.epi;
 		e.d1 = sp == 0 ? dA : dB;
 		e.d2 = sp == 0 ? dB : dA;
@@ -1126,7 +1143,7 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) ==
 	    hipSuccess || (set_err("stream capture failed"), false));
 	int rc = pt_launch_batch(form, b->nset, b->d_segs, b->d_begin, nseg,
-	    b->total, b->stream);
+	    b->total, g_nslot, b->stream);
 	if (rc == 0)
 		rc = pt_launch_reduce_batch(b->d_segs, b->d_begin, nseg, b->d_out,
 		    b->stream);

@@ -304,6 +304,34 @@ brsrc(const void *base, int64_t bytes)
 
 #define PT_OOR 0x80000000u  /* an offset every descriptor rejects */
 
+/* buffer store / load of one accumulator value at lane offset vofs plus a
+ * constant offset sofs: the constant goes to a scalar register, so a run of
+ * them costs no vector registers for addresses */
+template <class AE>
+__device__ __forceinline__ void
+bstore_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs, AE v)
+{
+	typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+	if constexpr (sizeof(AE) == 8)
+		__builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r,
+		    (int)vofs, sofs, 0);
+	else
+		__builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t,
+		    v), r, (int)vofs, sofs, 0);
+}
+
+template <class AE>
+__device__ __forceinline__ AE
+bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
+{
+	if constexpr (sizeof(AE) == 8)
+		return __builtin_bit_cast(AE,
+		    __builtin_amdgcn_raw_buffer_load_b64(r, (int)vofs, sofs, 0));
+	else
+		return __builtin_bit_cast(AE,
+		    __builtin_amdgcn_raw_buffer_load_b32(r, (int)vofs, sofs, 0));
+}
+
 
 /*
  * FP64 / FP32 staging: NSLOT slots, each holding one step (term, kappa
@@ -540,22 +568,30 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 
 /* -------------------------------------------------------------- kernel */
 
-/* workgroups per CU the register allocation targets: two where one
- * workgroup's LDS (two slots, 76 KB) and registers (<= 256 per lane) allow,
- * so that one workgroup's staging and barriers overlap the other's MFMAs */
-template <int F, int NSET> struct Occ {
-	static constexpr int WG_PER_CU = F == PT_FORM_FP64 && NSET == 2 ? 1 : 2;
-};
+/* c tile width of a form and W-set count (as pt_tcw) */
+constexpr int
+tcw_of(int F, int NSET)
+{
+	return (F == PT_FORM_FP32 && NSET == 1) ? 32 : 16;
+}
+
+/* Every tile kernel targets two workgroups per CU (PT_WG_PER_CU): one
+ * workgroup's LDS (two slots, 76 KB) and registers (<= 256 per lane) allow
+ * it, and one workgroup's staging and barriers then overlap the other's
+ * MFMAs.  Two W sets keep one set of accumulators: W of set 0 waits in
+ * the workgroup's save area while set 1 is formed. */
 
 /*
  * One work item (ijk, tile) of a spin case: everything between reading the
  * item and writing its energy to partial[it].  KIND is PT_AAA, PT_AAB, or
  * -1 to take the spin case from p.kind at run time (batched launches that
- * mix both cases; only the epilogue depends on it).
+ * mix both cases; only the epilogue depends on it).  With two W sets the
+ * sets are formed one after the other; wsv is the calling workgroup's save
+ * area for W of set 0 (4 TCW values per lane per wave).
  */
 template <int F, int KIND, int NSET>
 __device__ __forceinline__ void
-tile_item(const pt_launch &p, int64_t it)
+tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 {
 	typedef Form<F> FM;
 	typedef typename FM::G G;
@@ -564,14 +600,15 @@ tile_item(const pt_launch &p, int64_t it)
 	typedef typename FM::ACC ACC;
 	typedef typename FM::ER ER;
 	constexpr int KC = FM::KC;
-	constexpr int NCB = (F == PT_FORM_FP32 && NSET == 1) ? 2 : 1; /* pt_tcw */
-	constexpr int TCW = 16 * NCB;
+	constexpr int TCW = tcw_of(F, NSET);
+	constexpr int NCB = TCW / 16;
 	constexpr int GB = 32;  /* bytes per packed kappa group and pair */
 	/* S elements per thread per step */
 	constexpr int SEPT = F == PT_FORM_FP64 ? 1 : 2;
 	constexpr int SMEM = Stg::BYTES;
 	static_assert(SMEM >= 4 * NSET * TCW * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
+	static_assert(NSET == 1 || NSET == 2, "one or two W sets");
 
 	__shared__ __attribute__((aligned(16))) char smem[SMEM];
 	__shared__ double red[4];
@@ -595,16 +632,22 @@ tile_item(const pt_launch &p, int64_t it)
 	const bool act = a_lo < na && b_lo < na && a_lo + 15 > b_lo &&
 	    (!same || b_lo + 15 > C0);
 
-	ACC acc[NSET][TCW];
+	ACC acc[TCW];
 #pragma unroll
-	for (int s = 0; s < NSET; ++s)
-#pragma unroll
-		for (int x = 0; x < TCW; ++x)
-			acc[s][x] = ACC{0, 0, 0, 0};
+	for (int x = 0; x < TCW; ++x)
+		acc[x] = ACC{0, 0, 0, 0};
+	/* save area of W set 0: value (r, x) of (wave, lane) at element
+	 * ((r TCW + x) 4 + wave) 64 + lane */
+	constexpr int WST = 4 * 64 * (int)sizeof(AE);
+	const __amdgpu_buffer_rsrc_t wr = brsrc(wsv, NSET == 2 ?
+	    (int64_t)4 * TCW * WST : 0);
+	const uint32_t wvo = (uint32_t)((wave * 64 + lane) * sizeof(AE));
 
 	/* the term table, copied once into LDS: a term change then reads it
-	 * with LDS loads instead of waiting on global memory */
-	const int nterm = uni(p.nterm1 + p.nterm2);
+	 * with LDS loads instead of waiting on global memory.  Terms of one W
+	 * set: role a, role b, role c; set 1 follows set 0. */
+	const int nset_terms = uni(p.nterm1 + p.nterm2);
+	const int nterm = NSET * nset_terms;
 	{
 		const uint32_t *src = (const uint32_t *)p.terms;
 		uint32_t *dst = (uint32_t *)sterm;
@@ -625,8 +668,6 @@ tile_item(const pt_launch &p, int64_t it)
 	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
 	auto transpose = [&](AE *scratch) {
 #pragma unroll
-		for (int s = 0; s < NSET; ++s)
-#pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
 		for (int r = 0; r < 4; ++r) {
@@ -634,11 +675,11 @@ tile_item(const pt_launch &p, int64_t it)
 			__syncthreads();
 #pragma unroll
 			for (int c = 0; c < 16; ++c)
-				X[(lg * 16 + ln) * 17 + c] = acc[s][16 * cb + c][r];
+				X[(lg * 16 + ln) * 17 + c] = acc[16 * cb + c][r];
 			__syncthreads();
 #pragma unroll
 			for (int b = 0; b < 16; ++b)
-				acc[s][16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
+				acc[16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
 		}
 		__syncthreads();
 	};
@@ -763,12 +804,12 @@ tile_item(const pt_launch &p, int64_t it)
 		}
 
 		/* compute walker: term ct, chunk ck of step n */
-		int ct = 0, ck = 0, cK = uni(sterm[0].K), cset = uni(sterm[0].set);
+		int ct = 0, ck = 0, cK = uni(sterm[0].K);
 		int n = 0;
-		auto run_phase = [&](auto ph) {
+		/* steps of phase PH up to term tend (exclusive) */
+		auto run_phase = [&](auto ph, int tend) {
 			constexpr int PH = decltype(ph)::value;
 			constexpr bool TA = PH < 2 && TCW == 16;
-			const int tend = PH == 0 ? nta : PH == 1 ? nt1 : nterm;
 			while (ct < tend) {
 				/* loads of step n + D into the slot step n - 1 used */
 				if (pn < N) {
@@ -776,20 +817,11 @@ tile_item(const pt_launch &p, int64_t it)
 				}
 				const char *slot = smem + (n % NS) * Stg::SLOT;
 				const char *cS = slot, *cQ = slot + Stg::S_BYTES;
-				if constexpr (PH < 2) {
-					if (NSET == 1 || cset == 0)
-						mma_p1<F, TCW, PH == 0, TA>(acc[0], cS, cQ, wa, wb,
-						    ln, lg);
-					else
-						mma_p1<F, TCW, PH == 0, TA>(acc[NSET - 1], cS, cQ,
-						    wa, wb, ln, lg);
-				} else {
-					if (NSET == 1 || cset == 0)
-						mma_p2<F, TCW>(acc[0], cS, cQ, wa, wb, ln, lg);
-					else
-						mma_p2<F, TCW>(acc[NSET - 1], cS, cQ, wa, wb, ln,
-						    lg);
-				}
+				if constexpr (PH < 2)
+					mma_p1<F, TCW, PH == 0, TA>(acc, cS, cQ, wa, wb, ln,
+					    lg);
+				else
+					mma_p2<F, TCW>(acc, cS, cQ, wa, wb, ln, lg);
 				/* stage the S rows of step n + 1 */
 				if (n + 1 < N) {
 					wait_step(n + 1);
@@ -805,19 +837,33 @@ tile_item(const pt_launch &p, int64_t it)
 				ck += TA ? KSA : KSB;
 				if (ck >= cK) {
 					ck = 0;
-					if (++ct < nterm) {
+					if (++ct < nterm)
 						cK = uni(sterm[ct].K);
-						cset = uni(sterm[ct].set);
-					}
 				}
 			}
 		};
-		run_phase(std::integral_constant<int, 0>());
-		run_phase(std::integral_constant<int, 1>());
-		/* the slot of the last phase-1 step is free until step n issues
-		 * into it: transpose through it */
-		transpose((AE *)(smem + ((n + NS - 1) % NS) * Stg::SLOT));
-		run_phase(std::integral_constant<int, 2>());
+#pragma unroll 1
+		for (int s = 0; s < NSET; ++s) {
+			const int t0 = s * nset_terms;
+			run_phase(std::integral_constant<int, 0>(), t0 + nta);
+			run_phase(std::integral_constant<int, 1>(), t0 + nt1);
+			/* the slot of the last phase-1 step is free until step
+			 * n + D issues into it: transpose through it */
+			transpose((AE *)(smem + ((n + NS - 1) % NS) * Stg::SLOT));
+			run_phase(std::integral_constant<int, 2>(), t0 + nset_terms);
+			if (NSET == 2 && s == 0) {
+				/* W of set 0 to the save area, lane-contiguous per
+				 * (r, x, wave); the accumulators restart for set 1 */
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+#pragma unroll
+					for (int x = 0; x < TCW; ++x) {
+						bstore_ae<AE>(wr, wvo, (r * TCW + x) * WST,
+						    acc[x][r]);
+						acc[x][r] = 0;
+					}
+			}
+		}
 		(void)t;
 		(void)kc;
 		(void)cur;
@@ -832,10 +878,12 @@ tile_item(const pt_launch &p, int64_t it)
 	for (int r = 0; r < 4; ++r) {
 		__syncthreads();
 #pragma unroll
-		for (int s = 0; s < NSET; ++s)
-#pragma unroll
-			for (int x = 0; x < TCW; ++x)
-				E[(s * TCW + x) * 64] = acc[s][x][r];
+		for (int x = 0; x < TCW; ++x) {
+			E[x * 64] = acc[x][r];
+			if (NSET == 2)
+				E[(TCW + x) * 64] = bload_ae<AE>(wr, wvo,
+				    (r * TCW + x) * WST);
+		}
 		if (!act)
 			continue;
 		const int arow = FM::ROW_F64 ? (lg + 4 * r) : (4 * lg + r);
@@ -869,7 +917,7 @@ tile_item(const pt_launch &p, int64_t it)
 					w *= (ER)p.inv_scale[0];
 				term = w * (w - z) / dn;
 			} else {
-				ER wl = (ER)E[x * 64], wr = (ER)E[(TCW + x) * 64];
+				ER wl = (ER)E[(TCW + x) * 64], wr = (ER)E[x * 64];
 				if (FM::SPLIT) {
 					wl *= (ER)p.inv_scale[0];
 					wr *= (ER)p.inv_scale[1];
@@ -892,35 +940,50 @@ tile_item(const pt_launch &p, int64_t it)
 		p.partial[it] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-/* one launch over items [item0, item0 + n_chunk) of one spin case.  XCD-
- * aware order: workgroup n runs on XCD n % 8, and each XCD gets a
- * contiguous run of items, so neighbouring tiles of one ijk share L2. */
-template <int F, int KIND, int NSET>
-__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
-k_tiles(pt_launch p)
+/* save area of workgroup `slot` (two W sets; pt_wsave_bytes each) */
+template <int F>
+__device__ __forceinline__ typename Form<F>::AE *
+wsave_of(const pt_launch &p, int64_t slot)
 {
-	const int64_t per = (p.n_chunk + 7) / 8;
-	const int64_t it = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-	if (it >= p.n_chunk)
-		return;
-	tile_item<F, KIND, NSET>(p, p.item0 + it);
+	constexpr int64_t bytes = (int64_t)4 * tcw_of(F, 2) * 4 * 64 *
+	    sizeof(typename Form<F>::AE);
+	return (typename Form<F>::AE *)((char *)p.wsave + slot * bytes);
 }
 
 /*
- * One launch over the items of many segments (a segment = one spin case of
- * one system, both kinds mixed).  Segment s owns global items
- * [begin[s], begin[s + 1]); workgroup g takes global item g0 + g, so the
- * hardware dispatcher hands items out in segment order (the host sorts
- * segments by cost per item, largest first).
+ * One launch over items [item0, item0 + n_chunk) of one spin case.  XCD-
+ * aware order: workgroup n runs on XCD n % 8, and each XCD gets a
+ * contiguous run of items, so neighbouring tiles of one ijk share L2.
+ * One W set: one workgroup per item.  Two W sets: at most p.nslot
+ * workgroups (a multiple of 8), each owning save area blockIdx.x and
+ * walking its XCD's run with stride gridDim.x / 8.
  */
-template <int F, int NSET>
-__global__ __launch_bounds__(PT_THREADS, (Occ<F, NSET>::WG_PER_CU)) void
-k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t g0, int64_t n)
+template <int F, int KIND, int NSET>
+__global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
+k_tiles(pt_launch p)
 {
-	if ((int64_t)blockIdx.x >= n)
-		return;
-	const int64_t g = g0 + blockIdx.x;
+	const int64_t per = (p.n_chunk + 7) / 8;
+	const int64_t x = blockIdx.x & 7;
+	if constexpr (NSET == 1) {
+		const int64_t it = x * per + (blockIdx.x >> 3);
+		if (it < p.n_chunk)
+			tile_item<F, KIND, 1>(p, p.item0 + it, nullptr);
+	} else {
+		typename Form<F>::AE *wsv = wsave_of<F>(p, blockIdx.x);
+		const int64_t L = gridDim.x >> 3;
+		for (int64_t m = blockIdx.x >> 3; m < per; m += L) {
+			const int64_t it = x * per + m;
+			if (it >= p.n_chunk)
+				break;
+			tile_item<F, KIND, 2>(p, p.item0 + it, wsv);
+		}
+	}
+}
+
+/* the segment holding global item g: the last s with begin[s] <= g */
+__device__ __forceinline__ int
+seg_of(const int64_t *begin, int nseg, int64_t g)
+{
 	int lo = 0, hi = nseg - 1;
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
@@ -929,7 +992,36 @@ k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
 		else
 			hi = mid - 1;
 	}
-	tile_item<F, -1, NSET>(segs[lo], g - begin[lo]);
+	return lo;
+}
+
+/*
+ * One launch over the items of many segments (a segment = one spin case of
+ * one system, both kinds mixed).  Segment s owns global items
+ * [begin[s], begin[s + 1]); the host sorts segments by cost per item,
+ * largest first.  One W set: workgroup g takes global item g0 + g, so the
+ * hardware dispatcher hands items out in segment order.  Two W sets: at
+ * most nslot workgroups, workgroup g takes items g0 + g, g0 + g + grid, ...
+ * with save area g of each segment's plan.
+ */
+template <int F, int NSET>
+__global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
+k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
+    int64_t g0, int64_t n)
+{
+	if constexpr (NSET == 1) {
+		if ((int64_t)blockIdx.x >= n)
+			return;
+		const int64_t g = g0 + blockIdx.x;
+		const int s = seg_of(begin, nseg, g);
+		tile_item<F, -1, 1>(segs[s], g - begin[s], nullptr);
+	} else {
+		for (int64_t g = g0 + blockIdx.x; g < g0 + n; g += gridDim.x) {
+			const int s = seg_of(begin, nseg, g);
+			tile_item<F, -1, 2>(segs[s], g - begin[s],
+			    wsave_of<F>(segs[s], blockIdx.x));
+		}
+	}
 }
 
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
@@ -1103,7 +1195,14 @@ template <int F, int NSET>
 int
 launch_kind(const pt_launch *p, hipStream_t s)
 {
-	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
+	int64_t g = 8 * ((p->n_chunk + 7) / 8);
+	if (NSET == 2) {
+		/* one save area per workgroup */
+		if (!p->wsave || p->nslot < 8 || p->nslot % 8 != 0)
+			return (int)hipErrorInvalidValue;
+		g = g < p->nslot ? g : p->nslot;
+	}
+	dim3 grid((unsigned)g), block(PT_THREADS);
 	if (p->kind == PT_AAA)
 		k_tiles<F, PT_AAA, NSET><<<grid, block, 0, s>>>(*p);
 	else
@@ -1114,8 +1213,17 @@ launch_kind(const pt_launch *p, hipStream_t s)
 template <int F, int NSET>
 int
 launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t total, hipStream_t s)
+    int64_t total, int nslot, hipStream_t s)
 {
+	if (NSET == 2) {
+		/* persistent: one save area per workgroup */
+		if (nslot < 1)
+			return (int)hipErrorInvalidValue;
+		const int64_t g = total < nslot ? total : nslot;
+		k_tiles_batch<F, 2><<<dim3((unsigned)g), dim3(PT_THREADS), 0,
+		    s>>>(segs, begin, nseg, 0, total);
+		return (int)hipGetLastError();
+	}
 	/* grids of at most 2^22 workgroups */
 	const int64_t chunk = (int64_t)1 << 22;
 	for (int64_t g0 = 0; g0 < total; g0 += chunk) {
@@ -1154,14 +1262,14 @@ launch_form(const pt_launch *p, hipStream_t s)
 template <int F>
 int
 launch_batch_form(int nset, const pt_launch *segs, const int64_t *begin,
-    int nseg, int64_t total, hipStream_t s)
+    int nseg, int64_t total, int nslot, hipStream_t s)
 {
 	if (nset == 1) {
 		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
-			return launch_batch<F, 1>(segs, begin, nseg, total, s);
+			return launch_batch<F, 1>(segs, begin, nseg, total, nslot, s);
 	} else {
 		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
-			return launch_batch<F, 2>(segs, begin, nseg, total, s);
+			return launch_batch<F, 2>(segs, begin, nseg, total, nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1183,18 +1291,19 @@ pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *stream)
+    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	switch (form) {
 	case PT_FORM_FP64:
-		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total, s);
+		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
+		    nslot, s);
 	case PT_FORM_BF16X3:
 		return launch_batch_form<PT_FORM_BF16X3>(nset, segs, begin, nseg,
-		    total, s);
+		    total, nslot, s);
 	case PT_FORM_F16X3:
 		return launch_batch_form<PT_FORM_F16X3>(nset, segs, begin, nseg,
-		    total, s);
+		    total, nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1216,21 +1325,21 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *stream)
+    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	if (nseg <= 0 || total <= 0)
 		return 0;
 	if (agpr_unit(form, nset))
 		return pt_launch_batch_agpr(form, nset, segs, begin, nseg, total,
-		    stream);
+		    nslot, stream);
 	switch (form) {
 	case PT_FORM_FP64:
 		return launch_batch_form<PT_FORM_FP64>(nset, segs, begin, nseg,
-		    total, s);
+		    total, nslot, s);
 	case PT_FORM_FP32:
 		return launch_batch_form<PT_FORM_FP32>(nset, segs, begin, nseg,
-		    total, s);
+		    total, nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }

@@ -191,15 +191,26 @@ pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
 int
 pt_terms_order(std::vector<pt_term> &terms, int *n_role_a)
 {
-	/* phase 1: single index a, then single index b; phase 2: single
-	 * index c.  Each group keeps the table order. */
-	std::vector<pt_term> ga, gb, gc;
+	/* W set 0, then W set 1; within a set role a, role b, role c, each
+	 * group in table order.  Both sets have the same structure, so the
+	 * counts of set 0 hold for every set. */
+	int nset = 1;
 	for (const pt_term &t : terms)
-		(t.role == PT_ROLE_A ? ga : t.role == PT_ROLE_B ? gb : gc).push_back(t);
-	terms = ga;
-	terms.insert(terms.end(), gb.begin(), gb.end());
-	terms.insert(terms.end(), gc.begin(), gc.end());
+		nset = t.set + 1 > nset ? t.set + 1 : nset;
+	std::vector<pt_term> out;
+	int na = 0, n1 = 0;
+	for (int s = 0; s < nset; ++s)
+		for (int role = PT_ROLE_A; role <= PT_ROLE_C; ++role)
+			for (const pt_term &t : terms)
+				if (t.set == s && t.role == role) {
+					out.push_back(t);
+					if (s == 0 && role == PT_ROLE_A)
+						++na;
+					if (s == 0 && role != PT_ROLE_C)
+						++n1;
+				}
+	terms = out;
 	if (n_role_a)
-		*n_role_a = (int)ga.size();
-	return (int)(ga.size() + gb.size());
+		*n_role_a = na;
+	return n1;
 }
