@@ -58,10 +58,11 @@ pt_kgroups(int form, int64_t K)
 }
 
 /* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
- * TCW = 32 only for FP32 (T): its 32 accumulator blocks (4 registers each)
- * and the operands of a step fit the 256 VGPRs.  FP64 blocks take 8
- * registers, (fT) holds two W sets, and the split forms keep hi / lo
- * operand pairs, so those use 16. */
+ * TCW = 32 for FP32: its 32 accumulator blocks (4 registers each) and the
+ * operands of a step fit the 256 VGPRs.  FP64 blocks take 8 registers and
+ * the split forms keep hi / lo operand pairs, so those use 16.  (fT) forms
+ * its two W sets one after the other in the same registers, so the width
+ * does not depend on the W-set count. */
 #define PT_TA 32
 #define PT_TB 32
 #define PT_THREADS 256
@@ -69,7 +70,8 @@ pt_kgroups(int form, int64_t K)
 static inline int
 pt_tcw(int form, int nset)
 {
-	return (form == 1 && nset == 1) ? 32 : 16;
+	(void)nset;
+	return form == 1 ? 32 : 16;
 }
 
 /*
@@ -116,21 +118,26 @@ struct pt_launch {
 	int64_t item0, n_chunk;      /* items [item0, item0 + n_chunk) */
 	double *partial;             /* energy of each item */
 	double inv_scale[2];         /* per W set: 1 / (operand scale product) */
-	void *wsave;                 /* two W sets: W of set 0 per workgroup
-	                                (pt_wsave_bytes each), else unused */
-	int32_t nslot, pad2_;        /* workgroups a launch may use (wsave
-	                                slots) */
+	void *wsave;                 /* two W sets: W of set 0 of each item
+	                                of the launch (pt_wsave_bytes each,
+	                                item item0 + n at slot n), else
+	                                unused */
+	int32_t nslot, pad2_;        /* save areas in wsave (>= n_chunk) */
 	struct pt_epi epi;
 };
 
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* persistent workgroups per launch: two per CU */
+/* workgroups per CU the tile kernels are built for */
 #define PT_WG_PER_CU 2
 
-/* bytes of one workgroup's saved W set: 4 waves x TCW blocks x 4 values x
- * 64 lanes */
+/* items per launch with two W sets: each item of a launch has its own save
+ * area, so this bounds the area (4 GB for FP64) */
+#define PT_FT_CHUNK 32768
+
+/* bytes of one item's saved W set: 4 waves x TCW blocks x 4 values x 64
+ * lanes */
 static inline int64_t
 pt_wsave_bytes(int form, int nset)
 {
@@ -145,12 +152,14 @@ int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
 /* batched launch: device array of nseg launches (partial, ijk, tiles,
  * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
  * every item of every segment in one grid (several above 2^22 items).
- * Two W sets: a persistent grid of at most nslot workgroups, workgroup g
- * using save area g of its segment's wsave */
+ * Two W sets: grids of at most nslot items, item g0 + n of a grid using
+ * save area n of wsave (pt_wsave_bytes each) */
 int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream);
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    void *stream);
 int pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream);
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,
     const int64_t *begin, int nseg, double *out, void *stream);

@@ -61,8 +61,6 @@ now_s()
 		}                                                            \
 	} while (0)
 
-int g_nslot = 8;
-
 bool
 dev_init()
 {
@@ -80,12 +78,6 @@ dev_init()
 		return false;
 	}
 	HCHK(hipSetDevice(dev), false);
-	int cus = 0;
-	HCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-	    dev), false);
-	/* persistent workgroups per launch: a multiple of 8 (one share per
-	 * XCD) */
-	g_nslot = ((cus > 0 ? cus : 8) * PT_WG_PER_CU + 7) / 8 * 8;
 	g_dev_ready = true;
 	return true;
 }
@@ -278,6 +270,10 @@ struct libpt_amd_plan {
 	double *d_out = nullptr;
 	double scale[4] = {1, 1, 1, 1};
 	double prep_s = 0;
+	/* two W sets: save areas of one launch's items, allocated at the
+	 * first run (wslots of pt_wsave_bytes each) */
+	void *wsave = nullptr;
+	int64_t wslots = 0;
 };
 
 namespace {
@@ -297,6 +293,7 @@ plan_free(libpt_amd_plan *pl)
 	}
 	for (void *p : pl->owned)
 		(void)hipFree(p);
+	(void)hipFree(pl->wsave);
 	(void)hipFree(pl->d_out);
 	if (pl->ev0)
 		(void)hipEventDestroy(pl->ev0);
@@ -512,13 +509,6 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	PFAIL(hipEventCreate(&pl->ev0) == hipSuccess &&
 	    hipEventCreate(&pl->ev1) == hipSuccess);
 	PFAIL(dev_alloc((void **)&pl->d_out, 4 * sizeof(double)));
-	/* two W sets: W of set 0 is saved per workgroup between the passes */
-	void *wsave = nullptr;
-	if (ft) {
-		PFAIL(dev_alloc(&wsave, (size_t)(g_nslot *
-		    pt_wsave_bytes(form, 2))));
-		pl->owned.push_back(wsave);
-	}
 
 	int64_t n[10];
 	const int narr = api_sizes(call, oa, va, ob, vb, n);
@@ -724,8 +714,6 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 			c.L.buf[sl + 11] = qT[s].p[mixC];
 			c.L.inv_scale[s] = 1.0 / (pl->scale[2 * s] * pl->scale[2 * s + 1]);
 		}
-		c.L.wsave = wsave;
-		c.L.nslot = g_nslot;
 		pt_epi &e = c.L.epi;
 		e.d1 = sp == 0 ? dA : dB;
 		e.d2 = sp == 0 ? dB : dA;
@@ -815,11 +803,26 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 	if (e <= b)
 		return 0;
 	const int64_t item_b = b * c.n_tiles, item_e = e * c.n_tiles;
-	const int64_t chunk = (int64_t)1 << 22;
+	const bool two = c.L.nset == 2;
+	if (two && !pl->wsave) {
+		/* one save area per item of a launch */
+		int64_t most = 1;
+		for (const Case &d : pl->cases)
+			most = std::max(most, d.n_ijk * d.n_tiles);
+		const int64_t slots = std::min<int64_t>(PT_FT_CHUNK, most);
+		if (!dev_alloc(&pl->wsave, (size_t)slots *
+		    (size_t)pt_wsave_bytes(pl->form, 2)))
+			return -1;
+		pl->wslots = slots;
+	}
+	/* grids of at most 2^22 items; two W sets: of at most wslots */
+	const int64_t chunk = two ? pl->wslots : (int64_t)1 << 22;
 	for (int64_t it = item_b; it < item_e; it += chunk) {
 		pt_launch L = c.L;
 		L.item0 = it;
 		L.n_chunk = item_e - it < chunk ? item_e - it : chunk;
+		L.wsave = pl->wsave;
+		L.nslot = (int32_t)pl->wslots;
 		int rc = pt_launch_tiles(pl->form, &L, pl->stream);
 		if (rc != 0) {
 			set_err("tile kernel launch failed: %s",
@@ -1017,6 +1020,9 @@ struct libpt_amd_batch {
 	pt_launch *d_segs = nullptr;
 	int64_t *d_begin = nullptr;
 	double *d_out = nullptr;
+	void *wsave = nullptr;       /* two W sets: one save area per item of
+	                                a grid, wslots of them */
+	int64_t wslots = 0;
 	int64_t total = 0;
 	double flop = 0;
 	hipGraph_t graph = nullptr;
@@ -1041,6 +1047,7 @@ batch_free(libpt_amd_batch *b)
 	(void)hipFree(b->d_segs);
 	(void)hipFree(b->d_begin);
 	(void)hipFree(b->d_out);
+	(void)hipFree(b->wsave);
 	if (b->ev0)
 		(void)hipEventDestroy(b->ev0);
 	if (b->ev1)
@@ -1130,6 +1137,11 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(dev_alloc((void **)&b->d_segs, sizeof(pt_launch) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_begin, sizeof(int64_t) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_out, sizeof(double) * (nseg + 1)));
+	if (b->nset == 2 && b->total > 0) {
+		b->wslots = std::min<int64_t>(PT_FT_CHUNK, b->total);
+		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots *
+		    (size_t)pt_wsave_bytes(form, 2)));
+	}
 	if (nseg > 0) {
 		BFAIL(hipMemcpy(b->d_segs, L.data(), sizeof(pt_launch) * nseg,
 		    hipMemcpyHostToDevice) == hipSuccess ||
@@ -1143,7 +1155,7 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) ==
 	    hipSuccess || (set_err("stream capture failed"), false));
 	int rc = pt_launch_batch(form, b->nset, b->d_segs, b->d_begin, nseg,
-	    b->total, g_nslot, b->stream);
+	    b->total, b->wsave, (int)b->wslots, b->stream);
 	if (rc == 0)
 		rc = pt_launch_reduce_batch(b->d_segs, b->d_begin, nseg, b->d_out,
 		    b->stream);

@@ -572,7 +572,7 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 constexpr int
 tcw_of(int F, int NSET)
 {
-	return (F == PT_FORM_FP32 && NSET == 1) ? 32 : 16;
+	return NSET > 0 && F == PT_FORM_FP32 ? 32 : 16;
 }
 
 /* Every tile kernel targets two workgroups per CU (PT_WG_PER_CU): one
@@ -940,23 +940,23 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		p.partial[it] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-/* save area of workgroup `slot` (two W sets; pt_wsave_bytes each) */
+/* save area `slot` of wsave (two W sets; pt_wsave_bytes each) */
 template <int F>
 __device__ __forceinline__ typename Form<F>::AE *
-wsave_of(const pt_launch &p, int64_t slot)
+wsave_of(void *wsave, int64_t slot)
 {
 	constexpr int64_t bytes = (int64_t)4 * tcw_of(F, 2) * 4 * 64 *
 	    sizeof(typename Form<F>::AE);
-	return (typename Form<F>::AE *)((char *)p.wsave + slot * bytes);
+	return (typename Form<F>::AE *)((char *)wsave + slot * bytes);
 }
 
 /*
  * One launch over items [item0, item0 + n_chunk) of one spin case.  XCD-
  * aware order: workgroup n runs on XCD n % 8, and each XCD gets a
  * contiguous run of items, so neighbouring tiles of one ijk share L2.
- * One W set: one workgroup per item.  Two W sets: at most p.nslot
- * workgroups (a multiple of 8), each owning save area blockIdx.x and
- * walking its XCD's run with stride gridDim.x / 8.
+ * One workgroup per item, handed out by the dispatcher in order; with two
+ * W sets item item0 + n keeps W of set 0 in save area n (n_chunk <=
+ * p.nslot).
  */
 template <int F, int KIND, int NSET>
 __global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
@@ -964,20 +964,10 @@ k_tiles(pt_launch p)
 {
 	const int64_t per = (p.n_chunk + 7) / 8;
 	const int64_t x = blockIdx.x & 7;
-	if constexpr (NSET == 1) {
-		const int64_t it = x * per + (blockIdx.x >> 3);
-		if (it < p.n_chunk)
-			tile_item<F, KIND, 1>(p, p.item0 + it, nullptr);
-	} else {
-		typename Form<F>::AE *wsv = wsave_of<F>(p, blockIdx.x);
-		const int64_t L = gridDim.x >> 3;
-		for (int64_t m = blockIdx.x >> 3; m < per; m += L) {
-			const int64_t it = x * per + m;
-			if (it >= p.n_chunk)
-				break;
-			tile_item<F, KIND, 2>(p, p.item0 + it, wsv);
-		}
-	}
+	const int64_t it = x * per + (blockIdx.x >> 3);
+	if (it < p.n_chunk)
+		tile_item<F, KIND, NSET>(p, p.item0 + it,
+		    NSET == 2 ? wsave_of<F>(p.wsave, it) : nullptr);
 }
 
 /* the segment holding global item g: the last s with begin[s] <= g */
@@ -998,30 +988,22 @@ seg_of(const int64_t *begin, int nseg, int64_t g)
 /*
  * One launch over the items of many segments (a segment = one spin case of
  * one system, both kinds mixed).  Segment s owns global items
- * [begin[s], begin[s + 1]); the host sorts segments by cost per item,
- * largest first.  One W set: workgroup g takes global item g0 + g, so the
- * hardware dispatcher hands items out in segment order.  Two W sets: at
- * most nslot workgroups, workgroup g takes items g0 + g, g0 + g + grid, ...
- * with save area g of each segment's plan.
+ * [begin[s], begin[s + 1]); workgroup g takes global item g0 + g, so the
+ * hardware dispatcher hands items out in segment order (the host sorts
+ * segments by cost per item, largest first).  Two W sets: save area g of
+ * wsave.
  */
 template <int F, int NSET>
 __global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
 k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t g0, int64_t n)
+    int64_t g0, int64_t n, void *wsave)
 {
-	if constexpr (NSET == 1) {
-		if ((int64_t)blockIdx.x >= n)
-			return;
-		const int64_t g = g0 + blockIdx.x;
-		const int s = seg_of(begin, nseg, g);
-		tile_item<F, -1, 1>(segs[s], g - begin[s], nullptr);
-	} else {
-		for (int64_t g = g0 + blockIdx.x; g < g0 + n; g += gridDim.x) {
-			const int s = seg_of(begin, nseg, g);
-			tile_item<F, -1, 2>(segs[s], g - begin[s],
-			    wsave_of<F>(segs[s], blockIdx.x));
-		}
-	}
+	if ((int64_t)blockIdx.x >= n)
+		return;
+	const int64_t g = g0 + blockIdx.x;
+	const int s = seg_of(begin, nseg, g);
+	tile_item<F, -1, NSET>(segs[s], g - begin[s],
+	    NSET == 2 ? wsave_of<F>(wsave, blockIdx.x) : nullptr);
 }
 
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
@@ -1195,14 +1177,10 @@ template <int F, int NSET>
 int
 launch_kind(const pt_launch *p, hipStream_t s)
 {
-	int64_t g = 8 * ((p->n_chunk + 7) / 8);
-	if (NSET == 2) {
-		/* one save area per workgroup */
-		if (!p->wsave || p->nslot < 8 || p->nslot % 8 != 0)
-			return (int)hipErrorInvalidValue;
-		g = g < p->nslot ? g : p->nslot;
-	}
-	dim3 grid((unsigned)g), block(PT_THREADS);
+	/* one save area per item */
+	if (NSET == 2 && (!p->wsave || p->n_chunk > p->nslot))
+		return (int)hipErrorInvalidValue;
+	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
 	if (p->kind == PT_AAA)
 		k_tiles<F, PT_AAA, NSET><<<grid, block, 0, s>>>(*p);
 	else
@@ -1213,23 +1191,20 @@ launch_kind(const pt_launch *p, hipStream_t s)
 template <int F, int NSET>
 int
 launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t total, int nslot, hipStream_t s)
+    int64_t total, void *wsave, int nslot, hipStream_t s)
 {
+	/* grids of at most 2^22 workgroups; two W sets: of at most nslot, one
+	 * save area per item */
+	int64_t chunk = (int64_t)1 << 22;
 	if (NSET == 2) {
-		/* persistent: one save area per workgroup */
-		if (nslot < 1)
+		if (!wsave || nslot < 1)
 			return (int)hipErrorInvalidValue;
-		const int64_t g = total < nslot ? total : nslot;
-		k_tiles_batch<F, 2><<<dim3((unsigned)g), dim3(PT_THREADS), 0,
-		    s>>>(segs, begin, nseg, 0, total);
-		return (int)hipGetLastError();
+		chunk = nslot;
 	}
-	/* grids of at most 2^22 workgroups */
-	const int64_t chunk = (int64_t)1 << 22;
 	for (int64_t g0 = 0; g0 < total; g0 += chunk) {
 		const int64_t n = total - g0 < chunk ? total - g0 : chunk;
 		k_tiles_batch<F, NSET><<<dim3((unsigned)n), dim3(PT_THREADS), 0,
-		    s>>>(segs, begin, nseg, g0, n);
+		    s>>>(segs, begin, nseg, g0, n, wsave);
 	}
 	return (int)hipGetLastError();
 }
@@ -1262,14 +1237,16 @@ launch_form(const pt_launch *p, hipStream_t s)
 template <int F>
 int
 launch_batch_form(int nset, const pt_launch *segs, const int64_t *begin,
-    int nseg, int64_t total, int nslot, hipStream_t s)
+    int nseg, int64_t total, void *wsave, int nslot, hipStream_t s)
 {
 	if (nset == 1) {
 		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
-			return launch_batch<F, 1>(segs, begin, nseg, total, nslot, s);
+			return launch_batch<F, 1>(segs, begin, nseg, total, wsave,
+			    nslot, s);
 	} else {
 		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
-			return launch_batch<F, 2>(segs, begin, nseg, total, nslot, s);
+			return launch_batch<F, 2>(segs, begin, nseg, total, wsave,
+			    nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1291,19 +1268,20 @@ pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream)
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	switch (form) {
 	case PT_FORM_FP64:
 		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
-		    nslot, s);
+		    wsave, nslot, s);
 	case PT_FORM_BF16X3:
 		return launch_batch_form<PT_FORM_BF16X3>(nset, segs, begin, nseg,
-		    total, nslot, s);
+		    total, wsave, nslot, s);
 	case PT_FORM_F16X3:
 		return launch_batch_form<PT_FORM_F16X3>(nset, segs, begin, nseg,
-		    total, nslot, s);
+		    total, wsave, nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1325,21 +1303,22 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, int nslot, void *stream)
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	if (nseg <= 0 || total <= 0)
 		return 0;
 	if (agpr_unit(form, nset))
 		return pt_launch_batch_agpr(form, nset, segs, begin, nseg, total,
-		    nslot, stream);
+		    wsave, nslot, stream);
 	switch (form) {
 	case PT_FORM_FP64:
 		return launch_batch_form<PT_FORM_FP64>(nset, segs, begin, nseg,
-		    total, nslot, s);
+		    total, wsave, nslot, s);
 	case PT_FORM_FP32:
 		return launch_batch_form<PT_FORM_FP32>(nset, segs, begin, nseg,
-		    total, nslot, s);
+		    total, wsave, nslot, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
