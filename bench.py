@@ -266,8 +266,11 @@ def main():
                          "profiled run equal to the timed launches")
     ap.add_argument("--tune-sample", type=int, default=8,
                     help="ijk triples per spin case per tuning candidate")
-    ap.add_argument("--configs", type=int, default=1,
-                    help="also run the other configurations (0: skip)")
+    ap.add_argument("--configs", type=int, default=0,
+                    help="also run the other configurations (small rpt, upt, "
+                    "rft, uft) in each form (1); off by default so that the "
+                    "kernel statistics of a profiled default run cover the "
+                    "metric's launches only")
     ap.add_argument("--batch", type=int, default=64,
                     help="systems in the batched-launch measurement (0: skip)")
     ap.add_argument("--cpu", default="16,160",
