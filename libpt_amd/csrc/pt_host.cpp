@@ -1356,8 +1356,24 @@ host_call(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
 		pl = libpt_amd_plan_create(call, form, oa, va, ob, vb, d);
 	if (pl) {
 		t2 = now_s();
-		for (int sc = 0; sc < (int)pl->cases.size() && ok; ++sc)
-			ok = libpt_amd_plan_run(pl, sc, 0, pl->cases[sc].n_ijk) == 0;
+		/* LIBPT_AMD_PROGRESS set: each spin case in 16 parts, with a line
+		 * on stderr after each (a synchronisation per part) */
+		const bool progress = getenv("LIBPT_AMD_PROGRESS") != nullptr;
+		const int64_t parts = progress ? 16 : 1;
+		for (int sc = 0; sc < (int)pl->cases.size() && ok; ++sc) {
+			const int64_t nijk = pl->cases[sc].n_ijk;
+			for (int64_t p = 0; p < parts && ok; ++p) {
+				const int64_t b = nijk * p / parts;
+				const int64_t en = nijk * (p + 1) / parts;
+				ok = libpt_amd_plan_run(pl, sc, b, en) == 0;
+				if (ok && progress) {
+					ok = libpt_amd_plan_sync(pl) == 0;
+					fprintf(stderr, "[libpt_amd] spin case %d: ijk %lld of "
+					    "%lld done, %.1f s\n", sc, (long long)en,
+					    (long long)nijk, now_s() - t2);
+				}
+			}
+		}
 		if (ok)
 			e = libpt_amd_plan_energy(pl);
 	}
