@@ -324,9 +324,9 @@ def main():
                           "s": time.perf_counter() - t0}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is "
               "not supported (the ijk work list would be split across GPUs "
-              "with one 8-byte sum, see DESIGN.md)", file=sys.stderr)
+              "with one 8-byte sum)", file=sys.stderr)
         sys.exit(2)
 
     from libpt_amd import api

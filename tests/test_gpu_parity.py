@@ -40,7 +40,7 @@ def test_fixture_fp32(golden, name):
 # (kind, dims, extra generator args): tile edges (v not a multiple of
 # 16/32), odd/even spin sizes, (fT) with omega folded into d_ov
 CASES = [
-    ("rpt", (8, 64), dict(sig_f=1e-2)),          # small BASELINE config
+    ("rpt", (8, 64), dict(sig_f=1e-2)),          # the small rpt workload
     ("rpt", (6, 37), dict(sig_f=1e-2)),
     ("rpt", (3, 50), dict()),
     ("upt", (5, 20, 4, 23), dict(sig_f=1e-2)),
@@ -56,7 +56,7 @@ def test_synthetic_fp64(kind, dims, kw):
     e = api.dropin(kind, dims, arr)
     eo = oracle.energy(kind, dims, arr)
     print(kind, dims, "gpu", e, "oracle", eo, "dE", e - eo)
-    # FP64 parity: 1e-10 Eh absolute at the small BASELINE size, and a
+    # FP64 parity: 1e-10 Eh absolute at the small rpt size (o=8, v=64), and a
     # relative bound that scales with |E| (summation order only)
     assert abs(e - eo) <= max(1e-10, 1e-13 * abs(eo))
 
