@@ -337,28 +337,38 @@ bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
  * FP64 / FP32 staging: NSLOT slots, each holding one step (term, kappa
  * chunk) of the tile, loaded NSLOT - 1 steps ahead of the step computed.
  * Slot (bytes):
- *   S: 32 rows x 80 B (the chunk's kappa of one row, then padding, so the
- *      16 rows one MFMA operand read touches fall in distinct bank groups);
- *   Q: 32 u-rows x 1040 B.  A u-row is the chunk of the tile's w range:
+ *   S: 32 rows of 64 B of data (the chunk's kappa of one row) + padding;
+ *   Q: 32 u-rows of 1 KB of data + padding.  A u-row is the chunk of the
+ *      tile's w range:
  *      type A (w range 16, roles a and b of a 16-wide c tile): 16 w x 64 B,
  *        i.e. two 32-byte packed kappa groups per (u, w);
  *      type B (w range 32: role c, and 32-wide c tiles): 32 w x 32 B, one
  *        packed group per (u, w).
  *      Either way a u-row is one 1 KB DMA run (lane l writes bytes 16 l ..
- *      16 l + 15), and the 16-byte pad puts consecutive u in distinct bank
- *      groups.
+ *      16 l + 15).
  * MFMA lane group lg reads chunk lg of a type-A row (16 B: kappa 2lg, 2lg+1
  * for FP64, 4lg .. 4lg+3 for FP32) or element lg of a type-B row (8 B: kappa
- * lg for FP64, 2lg, 2lg+1 for FP32), the same mapping for both operands.
+ * lg for FP64, 2lg, 2lg+1 for FP32), the same mapping for both operands;
+ * lane n of a lane group reads row n.  The row strides are those at which
+ * such reads have no LDS bank conflicts on gfx950 (measured with
+ * SQ_LDS_BANK_CONFLICT): 16-byte reads over 16 rows need a stride of 32
+ * modulo 64 bytes (type A: S 96, Q 1056), 8-byte reads 16 modulo 64 (type
+ * B: S 80, Q 1040); the other strides cost a 2-way conflict.
  */
 struct Stg {
-	static constexpr int S_ROW = 80;
-	static constexpr int S_BYTES = 32 * S_ROW;
-	static constexpr int Q_USTR = 1024 + 16;
-	static constexpr int Q_BYTES = 32 * Q_USTR;
+	static constexpr int S_ROW_A = 96, S_ROW_B = 80;
+	static constexpr int Q_USTR_A = 1024 + 32, Q_USTR_B = 1024 + 16;
+	static constexpr int S_BYTES = 32 * S_ROW_A;
+	static constexpr int Q_BYTES = 32 * Q_USTR_A;
 	static constexpr int SLOT = S_BYTES + Q_BYTES;
 	static constexpr int NSLOT = 2;
 	static constexpr int BYTES = NSLOT * SLOT;
+	template <bool TA> static constexpr int s_row() {
+		return TA ? S_ROW_A : S_ROW_B;
+	}
+	template <bool TA> static constexpr int q_ustr() {
+		return TA ? Q_USTR_A : Q_USTR_B;
+	}
 };
 
 #define PT_QRUN_W 8   /* DMA runs per wave per step (32 u-rows / 4 waves) */
@@ -396,7 +406,7 @@ issue_q(const Step &s, int g, char *sQ, int wave,
 	for (int n = 0; n < PT_QRUN_W; ++n)
 		__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
 		    (__attribute__((address_space(3))) void *)(sQ +
-		    (wave + 4 * n) * Stg::Q_USTR), 16, vo[n], 0, 0, 0);
+		    (wave + 4 * n) * Stg::q_ustr<TA>()), 16, vo[n], 0, 0, 0);
 }
 
 /* wait until at most n vector-memory loads of this wave are outstanding */
@@ -421,7 +431,8 @@ s_lane(const Step &s, int tid, bool ta, uint32_t &ofs, int &lofs)
 	const int sg = s.s0 + row;
 	ofs = (row < s.sn && sg < s.sext) ?
 	    (uint32_t)(((int64_t)sg * s.s_st + k0) * ES) : PT_OOR;
-	lofs = row < 32 ? row * Stg::S_ROW + k0 * ES : -1;
+	lofs = row < 32 ? row * (ta ? Stg::S_ROW_A : Stg::S_ROW_B) + k0 * ES :
+	    -1;
 }
 
 template <int F, int SEPT>
@@ -497,8 +508,8 @@ mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 	typedef typename Form<F>::L L;
 	constexpr int NB = TA ? 16 : 8, WB = TA ? 64 : 32;
 	const int ws = RA ? wa : wb, wu = RA ? wb : wa;
-	const char *pf = sS + (16 * ws + ln) * Stg::S_ROW + NB * lg;
-	const char *pv = sQ + (16 * wu + ln) * Stg::Q_USTR + NB * lg;
+	const char *pf = sS + (16 * ws + ln) * Stg::s_row<TA>() + NB * lg;
+	const char *pv = sQ + (16 * wu + ln) * Stg::q_ustr<TA>() + NB * lg;
 	if constexpr (Form<F>::SPLIT) {
 		const v4u f = split_rd<TA>(pf);
 #pragma unroll
@@ -533,12 +544,14 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 {
 	typedef typename Form<F>::L L;
 	constexpr int NCB = TCW / 16;
-	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR + 16 * wb * 32 + 8 * lg;
+	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR_B + 16 * wb * 32 +
+	    8 * lg;
 	if constexpr (Form<F>::SPLIT) {
 		v4u f[NCB];
 #pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
-			f[cb] = split_rd<false>(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
+			f[cb] = split_rd<false>(sS + (16 * cb + ln) * Stg::S_ROW_B +
+			    8 * lg);
 #pragma unroll
 		for (int b = 0; b < 16; ++b) {
 			const v4u q = split_rd<false>(pv + 32 * b);
@@ -552,7 +565,8 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 		V f[NCB];
 #pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
-			f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW + 8 * lg);
+			f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW_B +
+			    8 * lg);
 #pragma unroll
 		for (int b = 0; b < 16; ++b) {
 			const V q = *(const V *)(pv + 32 * b);
