@@ -475,9 +475,31 @@ family_scale(libpt_amd_plan *pl, const double *const *arr, const int64_t *n,
 
 } /* namespace */
 
-extern "C" libpt_amd_plan *
-libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
-    size_t vb_, const void *const *arr)
+namespace {
+
+/* two W sets: the save areas of one launch's items, sized to the largest
+ * spin case (at most PT_FT_CHUNK); no-op when present */
+bool
+plan_wsave(libpt_amd_plan *pl)
+{
+	if (pl->wsave)
+		return true;
+	int64_t most = 1;
+	for (const Case &d : pl->cases)
+		most = std::max(most, d.n_ijk * d.n_tiles);
+	const int64_t slots = std::min<int64_t>(PT_FT_CHUNK, most);
+	if (!dev_alloc(&pl->wsave, (size_t)slots *
+	    (size_t)pt_wsave_bytes(pl->form, 2)))
+		return false;
+	pl->wslots = slots;
+	return true;
+}
+
+/* own_wsave: allocate the (fT) save areas now, so that runs allocate
+ * nothing (plans owned by a batch use the batch's) */
+libpt_amd_plan *
+plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
+    size_t vb_, const void *const *arr, bool own_wsave)
 {
 	if (!dev_init())
 		return nullptr;
@@ -738,12 +760,23 @@ libpt_amd_plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 		}
 		pl->cases.push_back(c);
 	}
+	if (ft && own_wsave)
+		PFAIL(plan_wsave(pl));
 	/* list uploads and slot clears ran on the null stream; the plan's
 	 * stream does not wait for it */
 	PFAIL(hipDeviceSynchronize() == hipSuccess ||
 	    (set_err("hipDeviceSynchronize failed"), false));
 #undef PFAIL
 	return pl;
+}
+
+} /* namespace */
+
+extern "C" libpt_amd_plan *
+libpt_amd_plan_create(int call, int form, size_t oa, size_t va, size_t ob,
+    size_t vb, const void *const *arr)
+{
+	return plan_create(call, form, oa, va, ob, vb, arr, true);
 }
 
 extern "C" void
@@ -804,17 +837,8 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 		return 0;
 	const int64_t item_b = b * c.n_tiles, item_e = e * c.n_tiles;
 	const bool two = c.L.nset == 2;
-	if (two && !pl->wsave) {
-		/* one save area per item of a launch */
-		int64_t most = 1;
-		for (const Case &d : pl->cases)
-			most = std::max(most, d.n_ijk * d.n_tiles);
-		const int64_t slots = std::min<int64_t>(PT_FT_CHUNK, most);
-		if (!dev_alloc(&pl->wsave, (size_t)slots *
-		    (size_t)pt_wsave_bytes(pl->form, 2)))
-			return -1;
-		pl->wslots = slots;
-	}
+	if (two && !plan_wsave(pl))
+		return -1;
 	/* grids of at most 2^22 items; two W sets: of at most wslots */
 	const int64_t chunk = two ? pl->wslots : (int64_t)1 << 22;
 	for (int64_t it = item_b; it < item_e; it += chunk) {
@@ -1097,8 +1121,9 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	std::vector<Seg> segs;
 	for (int s = 0; s < n; ++s) {
 		const int64_t *d = dims + 4 * s;
-		libpt_amd_plan *pl = libpt_amd_plan_create(call, form, (size_t)d[0],
-		    (size_t)d[1], (size_t)d[2], (size_t)d[3], dev_arrays + na * s);
+		libpt_amd_plan *pl = plan_create(call, form, (size_t)d[0],
+		    (size_t)d[1], (size_t)d[2], (size_t)d[3], dev_arrays + na * s,
+		    false);
 		BFAIL(pl != nullptr);
 		b->plans.push_back(pl);
 		for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
