@@ -122,7 +122,7 @@ struct pt_launch {
 	                                of the launch (pt_wsave_bytes each,
 	                                item item0 + n at slot n), else
 	                                unused */
-	int32_t nslot, pad2_;        /* save areas in wsave (>= n_chunk) */
+	int32_t nwsave, pad2_;        /* save areas in wsave (>= n_chunk) */
 	struct pt_epi epi;
 };
 
@@ -152,13 +152,13 @@ int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
 /* batched launch: device array of nseg launches (partial, ijk, tiles,
  * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
  * every item of every segment in one grid (several above 2^22 items).
- * Two W sets: grids of at most nslot items, item g0 + n of a grid using
+ * Two W sets: grids of at most nwsave items, item g0 + n of a grid using
  * save area n of wsave (pt_wsave_bytes each) */
 int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
     void *stream);
 int pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
     void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,

@@ -846,7 +846,7 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 		L.item0 = it;
 		L.n_chunk = item_e - it < chunk ? item_e - it : chunk;
 		L.wsave = pl->wsave;
-		L.nslot = (int32_t)pl->wslots;
+		L.nwsave = (int32_t)pl->wslots;
 		int rc = pt_launch_tiles(pl->form, &L, pl->stream);
 		if (rc != 0) {
 			set_err("tile kernel launch failed: %s",

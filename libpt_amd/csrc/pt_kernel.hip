@@ -675,10 +675,6 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		return make_step<ES, GB>(p, sterm[tn], i, j, k, A0, B0, C0, TCW);
 	};
 
-	/* steps = (term, kappa chunk) over all terms; phase 1 first */
-	int t = 0, kc = 0;
-	Step cur = step_of(0);
-
 	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
 	auto transpose = [&](AE *scratch) {
 #pragma unroll
@@ -716,7 +712,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 
 		/* prefetch walker: term pt, chunk pk of step pn (the next to issue) */
 		int pt = 0, pk = 0, pn = 0;
-		Step pre = cur;
+		Step pre = step_of(0);
 		bool pa = type_a(pre.role);
 		uint32_t vo[PT_QRUN_W], sofs;
 		int slofs;
@@ -878,9 +874,6 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 					}
 			}
 		}
-		(void)t;
-		(void)kc;
-		(void)cur;
 	}
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
@@ -970,7 +963,7 @@ wsave_of(void *wsave, int64_t slot)
  * contiguous run of items, so neighbouring tiles of one ijk share L2.
  * One workgroup per item, handed out by the dispatcher in order; with two
  * W sets item item0 + n keeps W of set 0 in save area n (n_chunk <=
- * p.nslot).
+ * p.nwsave).
  */
 template <int F, int KIND, int NSET>
 __global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
@@ -1192,7 +1185,7 @@ int
 launch_kind(const pt_launch *p, hipStream_t s)
 {
 	/* one save area per item */
-	if (NSET == 2 && (!p->wsave || p->n_chunk > p->nslot))
+	if (NSET == 2 && (!p->wsave || p->n_chunk > p->nwsave))
 		return (int)hipErrorInvalidValue;
 	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
 	if (p->kind == PT_AAA)
@@ -1205,15 +1198,15 @@ launch_kind(const pt_launch *p, hipStream_t s)
 template <int F, int NSET>
 int
 launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t total, void *wsave, int nslot, hipStream_t s)
+    int64_t total, void *wsave, int nwsave, hipStream_t s)
 {
-	/* grids of at most 2^22 workgroups; two W sets: of at most nslot, one
+	/* grids of at most 2^22 workgroups; two W sets: of at most nwsave, one
 	 * save area per item */
 	int64_t chunk = (int64_t)1 << 22;
 	if (NSET == 2) {
-		if (!wsave || nslot < 1)
+		if (!wsave || nwsave < 1)
 			return (int)hipErrorInvalidValue;
-		chunk = nslot;
+		chunk = nwsave;
 	}
 	for (int64_t g0 = 0; g0 < total; g0 += chunk) {
 		const int64_t n = total - g0 < chunk ? total - g0 : chunk;
@@ -1251,16 +1244,16 @@ launch_form(const pt_launch *p, hipStream_t s)
 template <int F>
 int
 launch_batch_form(int nset, const pt_launch *segs, const int64_t *begin,
-    int nseg, int64_t total, void *wsave, int nslot, hipStream_t s)
+    int nseg, int64_t total, void *wsave, int nwsave, hipStream_t s)
 {
 	if (nset == 1) {
 		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
 			return launch_batch<F, 1>(segs, begin, nseg, total, wsave,
-			    nslot, s);
+			    nwsave, s);
 	} else {
 		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
 			return launch_batch<F, 2>(segs, begin, nseg, total, wsave,
-			    nslot, s);
+			    nwsave, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1282,20 +1275,20 @@ pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
     void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	switch (form) {
 	case PT_FORM_FP64:
 		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
-		    wsave, nslot, s);
+		    wsave, nwsave, s);
 	case PT_FORM_BF16X3:
 		return launch_batch_form<PT_FORM_BF16X3>(nset, segs, begin, nseg,
-		    total, wsave, nslot, s);
+		    total, wsave, nwsave, s);
 	case PT_FORM_F16X3:
 		return launch_batch_form<PT_FORM_F16X3>(nset, segs, begin, nseg,
-		    total, wsave, nslot, s);
+		    total, wsave, nwsave, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1317,7 +1310,7 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 
 extern "C" int
 pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nslot,
+    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
     void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
@@ -1325,14 +1318,14 @@ pt_launch_batch(int form, int nset, const struct pt_launch *segs,
 		return 0;
 	if (agpr_unit(form, nset))
 		return pt_launch_batch_agpr(form, nset, segs, begin, nseg, total,
-		    wsave, nslot, stream);
+		    wsave, nwsave, stream);
 	switch (form) {
 	case PT_FORM_FP64:
 		return launch_batch_form<PT_FORM_FP64>(nset, segs, begin, nseg,
-		    total, wsave, nslot, s);
+		    total, wsave, nwsave, s);
 	case PT_FORM_FP32:
 		return launch_batch_form<PT_FORM_FP32>(nset, segs, begin, nseg,
-		    total, wsave, nslot, s);
+		    total, wsave, nwsave, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
