@@ -19,6 +19,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,8 +33,12 @@ namespace {
 
 void *(*g_malloc)(size_t) = malloc;
 void (*g_free)(void *) = free;
-char g_err[512];
-bool g_dev_ready = false;
+/* last error of the calling thread */
+thread_local char g_err[512];
+/* the device every call runs on (LIBPT_AMD_DEVICE, read once); the tune
+ * cache is shared by all threads */
+int g_dev = -1;
+std::mutex g_mu;
 
 void
 set_err(const char *fmt, ...)
@@ -61,24 +66,33 @@ now_s()
 		}                                                            \
 	} while (0)
 
+/* choose the device once per process, then make it current on the
+ * calling thread (HIP's current device is per thread, and plans, batches
+ * and their streams may be used from any thread) */
 bool
 dev_init()
 {
-	if (g_dev_ready)
-		return true;
-	int n = 0;
-	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
-		set_err("no HIP device available");
-		return false;
-	}
-	const char *e = getenv("LIBPT_AMD_DEVICE");
-	int dev = e ? atoi(e) : 0;
-	if (dev < 0 || dev >= n) {
-		set_err("LIBPT_AMD_DEVICE=%d out of range (%d devices)", dev, n);
-		return false;
+	int dev;
+	{
+		std::lock_guard<std::mutex> lk(g_mu);
+		if (g_dev < 0) {
+			int n = 0;
+			if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+				set_err("no HIP device available");
+				return false;
+			}
+			const char *e = getenv("LIBPT_AMD_DEVICE");
+			const int d = e ? atoi(e) : 0;
+			if (d < 0 || d >= n) {
+				set_err("LIBPT_AMD_DEVICE=%d out of range (%d devices)",
+				    d, n);
+				return false;
+			}
+			g_dev = d;
+		}
+		dev = g_dev;
 	}
 	HCHK(hipSetDevice(dev), false);
-	g_dev_ready = true;
 	return true;
 }
 
@@ -172,7 +186,8 @@ tile_block_env(TileBlock &tb)
  * Tuning cache: the blocking chosen by libpt_amd_plan_tune per (form,
  * call, oa, va, ob, vb), kept for the process and, when
  * LIBPT_AMD_TUNE_CACHE names a file, read from and appended to it (one
- * line per entry: form call oa va ob vb ga gb gc).
+ * line per entry: form call oa va ob vb ga gb gc).  Lookups and stores
+ * hold g_mu.
  */
 struct TuneEntry {
 	int form, call;
@@ -211,6 +226,7 @@ bool
 tune_lookup(int form, int call, int64_t oa, int64_t va, int64_t ob,
     int64_t vb, TileBlock &tb)
 {
+	std::lock_guard<std::mutex> lk(g_mu);
 	tune_load();
 	/* the latest entry wins */
 	for (size_t n = g_tune.size(); n-- > 0;) {
@@ -228,6 +244,7 @@ void
 tune_store(int form, int call, int64_t oa, int64_t va, int64_t ob,
     int64_t vb, const TileBlock &tb)
 {
+	std::lock_guard<std::mutex> lk(g_mu);
 	tune_load();
 	g_tune.push_back({form, call, oa, va, ob, vb, tb});
 	const char *path = getenv("LIBPT_AMD_TUNE_CACHE");
@@ -782,6 +799,7 @@ libpt_amd_plan_create(int call, int form, size_t oa, size_t va, size_t ob,
 extern "C" void
 libpt_amd_plan_destroy(libpt_amd_plan *pl)
 {
+	(void)dev_init();
 	plan_free(pl);
 }
 
@@ -822,6 +840,8 @@ libpt_amd_plan_factor(const libpt_amd_plan *pl, int sc)
 extern "C" int
 libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl || sc < 0 || sc >= (int)pl->cases.size()) {
 		set_err("bad plan or spin case");
 		return -1;
@@ -860,6 +880,8 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 extern "C" int
 libpt_amd_plan_sync(libpt_amd_plan *pl)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl)
 		return -1;
 	HCHK(hipStreamSynchronize(pl->stream), -1);
@@ -869,6 +891,8 @@ libpt_amd_plan_sync(libpt_amd_plan *pl)
 extern "C" double
 libpt_amd_plan_energy(libpt_amd_plan *pl)
 {
+	if (!dev_init())
+		return NAN;
 	if (!pl)
 		return NAN;
 	double e = 0.0;
@@ -892,6 +916,8 @@ libpt_amd_plan_energy(libpt_amd_plan *pl)
 extern "C" int
 libpt_amd_plan_ijk_energies(libpt_amd_plan *pl, int sc, double *out)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl || sc < 0 || sc >= (int)pl->cases.size())
 		return -1;
 	Case &c = pl->cases[sc];
@@ -927,6 +953,8 @@ libpt_amd_plan_ijk_list(const libpt_amd_plan *pl, int sc, int32_t *out)
 extern "C" int
 libpt_amd_plan_timer_begin(libpt_amd_plan *pl)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl)
 		return -1;
 	HCHK(hipEventRecord(pl->ev0, pl->stream), -1);
@@ -936,6 +964,8 @@ libpt_amd_plan_timer_begin(libpt_amd_plan *pl)
 extern "C" double
 libpt_amd_plan_timer_end_ms(libpt_amd_plan *pl)
 {
+	if (!dev_init())
+		return NAN;
 	if (!pl)
 		return NAN;
 	float ms = 0;
@@ -948,6 +978,8 @@ libpt_amd_plan_timer_end_ms(libpt_amd_plan *pl)
 extern "C" int
 libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl || ga <= 0 || gb <= 0 || gc <= 0) {
 		set_err("bad plan or blocking");
 		return -1;
@@ -973,6 +1005,8 @@ libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
 extern "C" int
 libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out3)
 {
+	if (!dev_init())
+		return -1;
 	if (!pl) {
 		set_err("null plan");
 		return -1;
@@ -1197,6 +1231,7 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 extern "C" void
 libpt_amd_batch_destroy(libpt_amd_batch *b)
 {
+	(void)dev_init();
 	batch_free(b);
 }
 
@@ -1216,6 +1251,8 @@ libpt_amd_batch_items(const libpt_amd_batch *b)
 extern "C" double
 libpt_amd_batch_last_ms(libpt_amd_batch *b)
 {
+	if (!dev_init())
+		return NAN;
 	if (!b)
 		return NAN;
 	float ms = 0;
@@ -1227,6 +1264,8 @@ libpt_amd_batch_last_ms(libpt_amd_batch *b)
 extern "C" int
 libpt_amd_batch_run(libpt_amd_batch *b)
 {
+	if (!dev_init())
+		return -1;
 	if (!b) {
 		set_err("null batch");
 		return -1;
@@ -1240,6 +1279,8 @@ libpt_amd_batch_run(libpt_amd_batch *b)
 extern "C" int
 libpt_amd_batch_energies(libpt_amd_batch *b, double *out)
 {
+	if (!dev_init())
+		return -1;
 	if (!b || !out) {
 		set_err("null batch or output");
 		return -1;
