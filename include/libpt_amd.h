@@ -136,7 +136,7 @@ int64_t libpt_amd_synth_size(const void *spec, int tensor);
  * for hardware traffic counters) */
 int libpt_amd_copy_kernel(void *dst, const void *src, int64_t n);
 
-/* message of the last failed call of this API */
+/* message of the calling thread's last failed call of this API */
 const char *libpt_amd_last_error(void);
 
 #ifdef __cplusplus
