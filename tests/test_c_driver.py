@@ -97,3 +97,43 @@ def test_rft_through_c(driver, tmp_path):
     assert e == api.dropin("rft", dims, arrays)
     assert abs(e - eo) <= max(1e-10, 1e-13 * abs(eo))
     assert n_malloc == n_free
+
+
+def _gpu_present():
+    from libpt_amd import api
+    try:
+        api.energy("rpt", "fp64", (3, 4), [np.zeros(n) for n in
+                                             (12, 12, 12, 288, 216, 288,
+                                              72 + 192)])
+        return True
+    except api.LibptError:
+        return False
+
+
+def test_dropin_without_gpu_exits_like_err(driver, tmp_path):
+    """No device: a pt.h call prints 'libpt: <reason>' and exits with status
+    1, as the reference's err(1, ...) does (src/pt.c:208-209)."""
+    if _gpu_present():
+        pytest.skip("a GPU is present")
+    arrays = [np.zeros(n) for n in (12, 12, 12, 288, 216, 288, 72 + 192)]
+    files = []
+    for n, a in enumerate(arrays):
+        p = tmp_path / f"a{n}.bin"
+        a.tofile(p)
+        files.append(str(p))
+    r = subprocess.run([driver, "rpt", "0", "3", "4", *files],
+                       capture_output=True, text=True, timeout=120)
+    print(r.returncode, r.stderr)
+    assert r.returncode == 1
+    assert r.stderr.startswith("libpt: ")
+
+
+def test_extension_api_without_gpu_returns_nan():
+    from libpt_amd import api
+    if _gpu_present():
+        pytest.skip("a GPU is present")
+    with pytest.raises(api.LibptError) as exc:
+        api.energy("rpt", "fp64", (3, 4), [np.zeros(n) for n in
+                                           (12, 12, 12, 288, 216, 288,
+                                            72 + 192)])
+    assert "device" in str(exc.value).lower()
