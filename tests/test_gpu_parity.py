@@ -142,3 +142,24 @@ def test_deterministic():
     e1 = api.dropin("rpt", (7, 45), arr)
     e2 = api.dropin("rpt", (7, 45), arr)
     assert e1 == e2
+
+
+@pytest.mark.parametrize("kind,dims,kw", [
+    ("rpt", (8, 64), dict(sig_f=1e-2)),
+    ("upt", (5, 20, 4, 23), dict(sig_f=1e-2)),
+    ("rft", (5, 30), dict(sig_f=1e-2, omega=0.3)),
+    ("uft", (4, 18, 3, 21), dict(sig_f=1e-2, omega=0.3)),
+])
+def test_fp64_against_compiled_reference(kind, dims, kw):
+    """The GPU against libpt itself (oracle/_ref, built from the reference's
+    sources).  libpt sums each thread's energies without compensation, so
+    its own result moves with the thread count: 6.4e-10 Eh between 1 and 2
+    threads at rpt o=8 v=64 (|E| ~ 2e3 Eh).  Bound: that spread plus the
+    oracle bound, 1e-9 Eh or 1e-12 relative."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    arr = _synth(kind, dims, seed=11, **kw)
+    e = api.dropin(kind, dims, arr)
+    er = oracle.ref_energy(kind, dims, arr)
+    print(kind, dims, "gpu", e, "reference", er, "dE", e - er)
+    assert abs(e - er) <= max(1e-9, 1e-12 * abs(er))
