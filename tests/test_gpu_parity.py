@@ -163,3 +163,23 @@ def test_fp64_against_compiled_reference(kind, dims, kw):
     er = oracle.ref_energy(kind, dims, arr)
     print(kind, dims, "gpu", e, "reference", er, "dE", e - er)
     assert abs(e - er) <= max(1e-9, 1e-12 * abs(er))
+
+
+@pytest.mark.parametrize("kind,dims,kw", [
+    ("rpt", (8, 64), dict(sig_f=1e-2)),
+    ("upt", (5, 20, 4, 23), dict(sig_f=1e-2)),
+    ("rft", (5, 30), dict(sig_f=1e-2, omega=0.3)),
+    ("uft", (4, 18, 3, 21), dict(sig_f=1e-2, omega=0.3)),
+])
+def test_fp32_against_compiled_reference(kind, dims, kw):
+    """_mp against libpt's _mp (sgemm, float W, Z and Dn, double sum):
+    both round W in float in different orders, so the bound is relative,
+    1e-5, as against the oracle."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    arr = [a.astype(np.float32) for a in _synth(kind, dims, seed=11, **kw)]
+    e = api.dropin(kind, dims, arr, mp=True)
+    er = oracle.ref_energy(kind, dims, arr, mp=True)
+    print(kind, dims, "gpu _mp", e, "reference _mp", er, "rel",
+          abs(e - er) / abs(er))
+    assert abs(e - er) <= 1e-5 * abs(er)
