@@ -248,16 +248,15 @@ def cpu_baseline(api, o, v, seed):
                       f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})"}
 
 
-def full_call(form, o, v, seed):
-    """One libpt_rpt-style call with host arrays: the inputs are generated in
+def full_call(form, kind, dims, seed, omega):
+    """One pt.h-style call with host arrays: the inputs are generated in
     HBM, copied to pageable host memory and freed on the device, then the call
     copies them back, packs its operands and runs every ijk triple.  Progress
     lines (one per 1/16 of a spin case) come from the library."""
     from libpt_amd import api
-    dims = (o, v)
-    spec = api.make_spec("rpt", dims, seed=seed)
+    spec = api.make_spec(kind, dims, seed=seed, omega=omega)
     t0 = time.perf_counter()
-    dev = api.synth_device("rpt", spec)
+    dev = api.synth_device(kind, spec)
     host = [api.from_device(p, n) for p, n in zip(dev.ptrs, dev.sizes)]
     dev.close()
     gb = sum(a.nbytes for a in host) / 1e9
@@ -265,11 +264,13 @@ def full_call(form, o, v, seed):
         f"{time.perf_counter() - t0:.1f} s")
     os.environ["LIBPT_AMD_PROGRESS"] = "1"
     t0 = time.perf_counter()
-    e, st = api.energy("rpt", form, dims, host, with_stats=True)
+    e, st = api.energy(kind, form, dims, host, with_stats=True)
     wall = time.perf_counter() - t0
     print(json.dumps({"full_call": {
-        "call": "libpt_rpt" if form == "fp64" else f"libpt_amd_energy(rpt, {form})",
-        "o": o, "v": v, "form": form, "host_input_gb": round(gb, 2),
+        "call": f"libpt_{kind}" if form == "fp64" else
+        f"libpt_amd_energy({kind}, {form})",
+        "dims": list(dims), "omega": omega, "form": form,
+        "host_input_gb": round(gb, 2),
         "wall_s": round(wall, 2), "h2d_s": round(st.h2d_s, 2),
         "prep_s": round(st.prep_s, 2), "kernel_s": round(st.kernel_s, 2),
         "flop": st.flop, "tflops_wall": round(st.flop / wall / 1e12, 3),
@@ -308,12 +309,22 @@ def main():
                     help="only run a device copy of this many GB (for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
     ap.add_argument("--full-call", default="",
-                    help="only time one whole host-array call of libpt_rpt "
-                         "(host to device copy included) at --o/--v in this "
-                         "form (fp64 is the drop-in libpt_rpt path) and exit")
+                    help="only time one whole host-array call (host to device "
+                         "copy included) in this form and exit; fp64 is the "
+                         "pt.h drop-in path")
+    ap.add_argument("--call", default="rpt",
+                    help="--full-call: rpt, upt, rft or uft")
+    ap.add_argument("--dims", default="",
+                    help="--full-call: o,v (restricted) or oa,va,ob,vb; "
+                         "default --o,--v")
+    ap.add_argument("--omega", type=float, default=0.3,
+                    help="--full-call of rft/uft: excitation energy (Eh)")
     args = ap.parse_args()
     if args.full_call:
-        full_call(args.full_call, args.o, args.v, args.seed)
+        dims = tuple(int(x) for x in args.dims.split(",")) if args.dims \
+            else (args.o, args.v)
+        full_call(args.full_call, args.call, dims, args.seed,
+                  args.omega if args.call.endswith("ft") else 0.0)
         return
     if args.copy_calibration > 0:
         from libpt_amd import api
