@@ -163,6 +163,19 @@ def run_batch(api, form, n, seed):
             wall = time.perf_counter() - t0
             ms = b.last_ms()
             flop, items = b.flop, b.items
+        # the same systems one after another, one plan each (plans made
+        # beforehand; the time covers their runs and energies only)
+        plans = [api.Plan("rpt", form, d, dev) for d, dev in systems]
+        try:
+            for pl in plans[:2]:
+                pl.run_all()
+                pl.energy()
+            t0 = time.perf_counter()
+            e_seq = [(pl.run_all(), pl.energy())[1] for pl in plans]
+            seq = time.perf_counter() - t0
+        finally:
+            for pl in plans:
+                pl.close()
     finally:
         for _, dev in systems:
             dev.close()
@@ -170,6 +183,9 @@ def run_batch(api, form, n, seed):
             "form": form, "items": items, "flop": flop,
             "prep_s": round(prep, 3), "graph_ms": round(ms, 3),
             "wall_s": round(wall, 4), "tflops": round(flop / ms / 1e9, 3),
+            "sequential_plans_s": round(seq, 4),
+            "speedup_vs_sequential": round(seq / wall, 3),
+            "same_energies_as_sequential": bool(np.array_equal(e, e_seq)),
             "E_sum": float(np.sum(e))}
 
 

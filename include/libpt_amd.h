@@ -102,9 +102,9 @@ typedef struct libpt_amd_batch libpt_amd_batch;
  * dims[4 s .. 4 s + 3] = (oa, va, ob, vb) of system s (restricted calls
  * read oa, va); dev_arrays[NA s + t] = its t-th pt.h array in device memory
  * (NA = 7 for rpt/upt, 10 for rft/uft).  Stages every system's operands
- * and records one graph: a single tile launch over every (system, spin
- * case, ijk, tile) item, largest items first, then one reduction per spin
- * case.  NULL on error. */
+ * and records one graph: the tile launches of every (system, spin case)
+ * in one chain, largest items first, then one reduction per spin case.
+ * NULL on error. */
 libpt_amd_batch *libpt_amd_batch_create(int call, int form, int n,
     const int64_t *dims, const void *const *dev_arrays);
 void libpt_amd_batch_destroy(libpt_amd_batch *batch);

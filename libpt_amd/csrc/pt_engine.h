@@ -149,17 +149,6 @@ int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
 /* the kernels built without VGPR-form MFMA (pt_kernel_agpr.hip): FP64
  * with two W sets and the split forms */
 int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
-/* batched launch: device array of nseg launches (partial, ijk, tiles,
- * terms, kind of each), begin[nseg + 1] prefix sums of their item counts;
- * every item of every segment in one grid (several above 2^22 items).
- * Two W sets: grids of at most nwsave items, item g0 + n of a grid using
- * save area n of wsave (pt_wsave_bytes each) */
-int pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
-    void *stream);
-int pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
-    void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,
     const int64_t *begin, int nseg, double *out, void *stream);

@@ -1079,7 +1079,7 @@ struct libpt_amd_batch {
 	int64_t *d_begin = nullptr;
 	double *d_out = nullptr;
 	void *wsave = nullptr;       /* two W sets: one save area per item of
-	                                a grid, wslots of them */
+	                                a launch, wslots of them */
 	int64_t wslots = 0;
 	int64_t total = 0;
 	double flop = 0;
@@ -1196,10 +1196,13 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(dev_alloc((void **)&b->d_segs, sizeof(pt_launch) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_begin, sizeof(int64_t) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_out, sizeof(double) * (nseg + 1)));
+	const int64_t wbytes = pt_wsave_bytes(form, 2);
 	if (b->nset == 2 && b->total > 0) {
-		b->wslots = std::min<int64_t>(PT_FT_CHUNK, b->total);
-		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots *
-		    (size_t)pt_wsave_bytes(form, 2)));
+		int64_t most = 1;
+		for (const Seg &g : segs)
+			most = std::max(most, g.items);
+		b->wslots = std::min<int64_t>(PT_FT_CHUNK, most);
+		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots * (size_t)wbytes));
 	}
 	if (nseg > 0) {
 		BFAIL(hipMemcpy(b->d_segs, L.data(), sizeof(pt_launch) * nseg,
@@ -1210,11 +1213,28 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 		    hipSuccess || (set_err("segment upload failed"), false));
 	}
 
-	/* the graph: batched tiles, then batched reduction */
+	/*
+	 * The graph: each segment's items as launches of the spin case's own
+	 * tile kernel (the one plans use), largest items first, in one chain,
+	 * then the batched reduction.  Every launch of a system of this size
+	 * range fills the GPU by itself: two concurrent branches (launches
+	 * dealt by accumulated cost) were measured 2-3 % slower than the chain,
+	 * which runs at the speed of the plans run one after another.
+	 */
 	BFAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) ==
 	    hipSuccess || (set_err("stream capture failed"), false));
-	int rc = pt_launch_batch(form, b->nset, b->d_segs, b->d_begin, nseg,
-	    b->total, b->wsave, (int)b->wslots, b->stream);
+	int rc = 0;
+	const int64_t chunk = b->nset == 2 ? b->wslots : (int64_t)1 << 22;
+	for (int g = 0; g < nseg && rc == 0; ++g) {
+		for (int64_t it = 0; it < segs[g].items && rc == 0; it += chunk) {
+			pt_launch Lc = L[g];
+			Lc.item0 = it;
+			Lc.n_chunk = std::min<int64_t>(chunk, segs[g].items - it);
+			Lc.wsave = b->wsave;
+			Lc.nwsave = (int32_t)b->wslots;
+			rc = pt_launch_tiles(form, &Lc, b->stream);
+		}
+	}
 	if (rc == 0)
 		rc = pt_launch_reduce_batch(b->d_segs, b->d_begin, nseg, b->d_out,
 		    b->stream);

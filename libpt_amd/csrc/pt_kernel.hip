@@ -597,11 +597,10 @@ tcw_of(int F, int NSET)
 
 /*
  * One work item (ijk, tile) of a spin case: everything between reading the
- * item and writing its energy to partial[it].  KIND is PT_AAA, PT_AAB, or
- * -1 to take the spin case from p.kind at run time (batched launches that
- * mix both cases; only the epilogue depends on it).  With two W sets the
- * sets are formed one after the other; wsv is the calling workgroup's save
- * area for W of set 0 (4 TCW values per lane per wave).
+ * item and writing its energy to partial[it].  KIND is the spin case,
+ * PT_AAA or PT_AAB (only the epilogue depends on it).  With two W sets the
+ * sets are formed one after the other; wsv is the item's save area for W
+ * of set 0 (4 TCW values per lane per wave).
  */
 template <int F, int KIND, int NSET>
 __device__ __forceinline__ void
@@ -633,7 +632,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	const int wa = wave & 1, wb = wave >> 1;
 	const int ln = lane & 15, lg = lane >> 4;
 
-	const bool same = (KIND >= 0 ? KIND : p.kind) == PT_AAA;
+	const bool same = KIND == PT_AAA;
 	const int64_t ijk_id = it / p.n_tiles;
 	const int tile = (int)(it - ijk_id * p.n_tiles);
 	const int i = p.ijk[3 * ijk_id], j = p.ijk[3 * ijk_id + 1];
@@ -977,42 +976,6 @@ k_tiles(pt_launch p)
 		    NSET == 2 ? wsave_of<F>(p.wsave, it) : nullptr);
 }
 
-/* the segment holding global item g: the last s with begin[s] <= g */
-__device__ __forceinline__ int
-seg_of(const int64_t *begin, int nseg, int64_t g)
-{
-	int lo = 0, hi = nseg - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (begin[mid] <= g)
-			lo = mid;
-		else
-			hi = mid - 1;
-	}
-	return lo;
-}
-
-/*
- * One launch over the items of many segments (a segment = one spin case of
- * one system, both kinds mixed).  Segment s owns global items
- * [begin[s], begin[s + 1]); workgroup g takes global item g0 + g, so the
- * hardware dispatcher hands items out in segment order (the host sorts
- * segments by cost per item, largest first).  Two W sets: save area g of
- * wsave.
- */
-template <int F, int NSET>
-__global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
-k_tiles_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t g0, int64_t n, void *wsave)
-{
-	if ((int64_t)blockIdx.x >= n)
-		return;
-	const int64_t g = g0 + blockIdx.x;
-	const int s = seg_of(begin, nseg, g);
-	tile_item<F, -1, NSET>(segs[s], g - begin[s],
-	    NSET == 2 ? wsave_of<F>(wsave, blockIdx.x) : nullptr);
-}
-
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
 __device__ __forceinline__ void
 reduce_block(const double *x, int64_t n, double *out)
@@ -1195,30 +1158,10 @@ launch_kind(const pt_launch *p, hipStream_t s)
 	return (int)hipGetLastError();
 }
 
-template <int F, int NSET>
-int
-launch_batch(const pt_launch *segs, const int64_t *begin, int nseg,
-    int64_t total, void *wsave, int nwsave, hipStream_t s)
-{
-	/* grids of at most 2^22 workgroups; two W sets: of at most nwsave, one
-	 * save area per item */
-	int64_t chunk = (int64_t)1 << 22;
-	if (NSET == 2) {
-		if (!wsave || nwsave < 1)
-			return (int)hipErrorInvalidValue;
-		chunk = nwsave;
-	}
-	for (int64_t g0 = 0; g0 < total; g0 += chunk) {
-		const int64_t n = total - g0 < chunk ? total - g0 : chunk;
-		k_tiles_batch<F, NSET><<<dim3((unsigned)n), dim3(PT_THREADS), 0,
-		    s>>>(segs, begin, nseg, g0, n, wsave);
-	}
-	return (int)hipGetLastError();
-}
-
 /* Kernels built in pt_kernel_agpr.hip, the same source compiled without
- * the VGPR-form MFMA option: FP64 with two W sets (its accumulators alone
- * fill 256 registers, and that option gives wrong energies there) and the
+ * the VGPR-form MFMA option: FP64 with two W sets (the option gave wrong
+ * energies when that kernel held both sets in registers; with the sets
+ * formed one after the other it is correct there but no faster) and the
  * split forms (whose accumulators stay in place either way, and for which
  * the option can crash the compiler). */
 constexpr bool
@@ -1241,23 +1184,6 @@ launch_form(const pt_launch *p, hipStream_t s)
 	return (int)hipErrorInvalidValue;
 }
 
-template <int F>
-int
-launch_batch_form(int nset, const pt_launch *segs, const int64_t *begin,
-    int nseg, int64_t total, void *wsave, int nwsave, hipStream_t s)
-{
-	if (nset == 1) {
-		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
-			return launch_batch<F, 1>(segs, begin, nseg, total, wsave,
-			    nwsave, s);
-	} else {
-		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
-			return launch_batch<F, 2>(segs, begin, nseg, total, wsave,
-			    nwsave, s);
-	}
-	return (int)hipErrorInvalidValue;
-}
-
 } /* namespace */
 
 #if PT_KERNEL_AGPR
@@ -1269,26 +1195,6 @@ pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
 	case PT_FORM_FP64: return launch_kind<PT_FORM_FP64, 2>(p, s);
 	case PT_FORM_BF16X3: return launch_form<PT_FORM_BF16X3>(p, s);
 	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
-	}
-	return (int)hipErrorInvalidValue;
-}
-
-extern "C" int
-pt_launch_batch_agpr(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
-    void *stream)
-{
-	hipStream_t s = (hipStream_t)stream;
-	switch (form) {
-	case PT_FORM_FP64:
-		return launch_batch<PT_FORM_FP64, 2>(segs, begin, nseg, total,
-		    wsave, nwsave, s);
-	case PT_FORM_BF16X3:
-		return launch_batch_form<PT_FORM_BF16X3>(nset, segs, begin, nseg,
-		    total, wsave, nwsave, s);
-	case PT_FORM_F16X3:
-		return launch_batch_form<PT_FORM_F16X3>(nset, segs, begin, nseg,
-		    total, wsave, nwsave, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1304,28 +1210,6 @@ pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 	switch (form) {
 	case PT_FORM_FP64: return launch_form<PT_FORM_FP64>(p, s);
 	case PT_FORM_FP32: return launch_form<PT_FORM_FP32>(p, s);
-	}
-	return (int)hipErrorInvalidValue;
-}
-
-extern "C" int
-pt_launch_batch(int form, int nset, const struct pt_launch *segs,
-    const int64_t *begin, int nseg, int64_t total, void *wsave, int nwsave,
-    void *stream)
-{
-	hipStream_t s = (hipStream_t)stream;
-	if (nseg <= 0 || total <= 0)
-		return 0;
-	if (agpr_unit(form, nset))
-		return pt_launch_batch_agpr(form, nset, segs, begin, nseg, total,
-		    wsave, nwsave, stream);
-	switch (form) {
-	case PT_FORM_FP64:
-		return launch_batch_form<PT_FORM_FP64>(nset, segs, begin, nseg,
-		    total, wsave, nwsave, s);
-	case PT_FORM_FP32:
-		return launch_batch_form<PT_FORM_FP32>(nset, segs, begin, nseg,
-		    total, wsave, nwsave, s);
 	}
 	return (int)hipErrorInvalidValue;
 }

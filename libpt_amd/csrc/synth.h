@@ -208,7 +208,7 @@ pt_ovvv_elem(const struct pt_synth_spec *s, uint32_t st, double sig, int blk,
     int64_t k, int64_t d, int64_t b, int64_t c)
 {
 	PT_NOCONTRACT
-	int64_t oa = s->oa, va = s->va, ob = s->ob, vb = s->vb;
+	int64_t va = s->va, vb = s->vb;
 	if (s->restricted) {
 		double x = pt_x4(s, st + 1, sig, va, va, va, k, d, b, c);
 		if (blk == 1)
