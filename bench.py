@@ -317,8 +317,12 @@ def main():
                     "rft, uft) in each form (1); off by default so that the "
                     "kernel statistics of a profiled default run cover the "
                     "metric's launches only")
-    ap.add_argument("--batch", type=int, default=64,
-                    help="systems in the batched-launch measurement (0: skip)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also time a batch of this many independent small "
+                    "systems (64: o in [4, 20], v in [40, 160]) as one graph "
+                    "against their plans run one after another; off by "
+                    "default, since its launches share the tile kernels and "
+                    "would mix into a profiled default run's statistics")
     ap.add_argument("--cpu", default="16,160",
                     help="o,v of the CPU-baseline sample; 'none' to skip")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
