@@ -1,8 +1,7 @@
 /*
  * Tile kernels built without the VGPR-form MFMA option: FP64 with two W
- * sets ((fT): 256 accumulator registers need the AGPR file) and the split
- * forms.  The same source as pt_kernel.hip; pt_launch_tiles and
- * pt_launch_batch there forward these launches here.
+ * sets ((fT)) and the split forms.  The same source as pt_kernel.hip;
+ * pt_launch_tiles there forwards these launches here.
  */
 #define PT_KERNEL_AGPR 1
 #include "pt_kernel.hip"
