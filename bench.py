@@ -275,6 +275,9 @@ def full_call(form, kind, dims, seed, omega):
     dev = api.synth_device(kind, spec)
     host = [api.from_device(p, n) for p, n in zip(dev.ptrs, dev.sizes)]
     dev.close()
+    if form == "fp32":
+        # the _mp drop-in takes float arrays: round them before the timer
+        host = [a.astype(np.float32) for a in host]
     gb = sum(a.nbytes for a in host) / 1e9
     log(f"[bench] full call: {gb:.1f} GB of host inputs ready in "
         f"{time.perf_counter() - t0:.1f} s")
@@ -283,8 +286,8 @@ def full_call(form, kind, dims, seed, omega):
     e, st = api.energy(kind, form, dims, host, with_stats=True)
     wall = time.perf_counter() - t0
     print(json.dumps({"full_call": {
-        "call": f"libpt_{kind}" if form == "fp64" else
-        f"libpt_amd_energy({kind}, {form})",
+        "call": f"libpt_{kind}" if form == "fp64" else f"libpt_{kind}_mp"
+        if form == "fp32" else f"libpt_amd_energy({kind}, {form})",
         "dims": list(dims), "omega": omega, "form": form,
         "host_input_gb": round(gb, 2),
         "wall_s": round(wall, 2), "h2d_s": round(st.h2d_s, 2),
@@ -302,7 +305,10 @@ def main():
     ap.add_argument("--o", type=int, default=45)
     ap.add_argument("--v", type=int, default=477)
     ap.add_argument("--nbatch", type=int, default=128)
-    ap.add_argument("--forms", default="fp64,bf16x3")
+    ap.add_argument("--forms", default="fp64,bf16x3",
+                    help="comma list of fp64, fp32 (the _mp form, float "
+                    "inputs), bf16x3, f16x3; the first is the headline and "
+                    "the others report their energy difference against it")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tune", type=int, default=0,
                     help="autotune the tile order before timing (1); the "
@@ -372,7 +378,16 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     res = {}
     for form in forms:
-        res[form] = run_form(api, form, dims, dev, args, args.nbatch)
+        if form == "fp32":
+            # the _mp form reads float inputs: the same generator, rounded
+            dev32 = api.synth_device("rpt", spec, fp32=True)
+            try:
+                res[form] = run_form(api, form, dims, dev32, args,
+                                     args.nbatch)
+            finally:
+                dev32.close()
+        else:
+            res[form] = run_form(api, form, dims, dev, args, args.nbatch)
         r = res[form]
         log(f"[bench] {form}: {r['tflops']:.2f} TFLOP/s, "
             f"{r['ms_per_step']:.1f} ms/step, dominant launch "
