@@ -39,7 +39,7 @@ def _alone(kind, form, d, dev):
 RPT_DIMS = [(6, 37), (3, 50), (8, 64), (2, 20), (1, 9), (5, 12), (4, 33)]
 
 
-@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("form", ["fp64", "fp32", "bf16x3", "f16x3"])
 def test_batch_rpt_matches_alone(form):
     systems = _systems("rpt", RPT_DIMS, fp32=form == "fp32", sig_f=1e-2)
     try:
@@ -91,6 +91,36 @@ def test_batch_other_calls(kind, dims_list, kw, form):
             tol = max(1e-10, 1e-13 * abs(eo)) if form == "fp64" else \
                 1e-5 * abs(eo)
             assert abs(e - eo) <= tol
+    finally:
+        _close(systems)
+
+
+# relative bounds of the split forms against the FP64 oracle (as in the
+# single-system parity tests; 5x for (fT), whose signed products cancel)
+SPLIT_TOL = {"bf16x3": 1e-4, "f16x3": 2e-6}
+
+
+@pytest.mark.parametrize("kind,dims_list,kw", [
+    ("upt", [(5, 20, 4, 23), (3, 9, 2, 11)], dict(sig_f=1e-2)),
+    ("rft", [(5, 30), (3, 40)], dict(sig_f=1e-2, omega=0.3)),
+    ("uft", [(4, 18, 3, 21), (3, 12, 4, 10)], dict(sig_f=1e-2, omega=0.3)),
+])
+@pytest.mark.parametrize("form", ["bf16x3", "f16x3"])
+def test_batch_split_forms(kind, dims_list, kw, form):
+    systems = _systems(kind, dims_list, **kw)
+    try:
+        with api.Batch(kind, form, [(d, dev) for d, _, dev in systems]) as b:
+            b.run()
+            eb = b.energies()
+        tol = SPLIT_TOL[form] * (5 if "ft" in kind else 1)
+        for (d, spec, dev), e in zip(systems, eb):
+            ea = _alone(kind, form, d, dev)
+            eo = oracle.energy(kind, d, api.synth_host(kind, spec))
+            rel = abs(e - eo) / abs(eo)
+            print(kind, form, d, "batch", e, "alone", ea, "oracle", eo,
+                  "rel", rel)
+            assert e == ea
+            assert rel <= tol
     finally:
         _close(systems)
 
