@@ -142,9 +142,11 @@ def run_form(api, form, dims, dev, args, nbatch):
     return res
 
 
-def run_batch(api, form, n, seed):
+def run_batch(api, form, n, seed, tune=0, sample=8):
     """n independent restricted systems (o in [4, 20], v in [40, 160]) in
-    one graph launch; one untimed replay first."""
+    one graph launch; one untimed replay first.  tune: autotune every
+    system's tile order first (each distinct size timed once); the plans
+    timed one after another then take the same orders."""
     dims = api.batch_dims(n, seed=seed)
     systems = [(d, api.synth_device("rpt", api.make_spec("rpt", d,
                                                          seed=seed + 1 + k),
@@ -154,6 +156,11 @@ def run_batch(api, form, n, seed):
         t0 = time.perf_counter()
         b = api.Batch("rpt", form, systems)
         prep = time.perf_counter() - t0
+        timed, tune_s = 0, 0.0
+        if tune:
+            t0 = time.perf_counter()
+            timed = b.tune(sample)
+            tune_s = time.perf_counter() - t0
         with b:
             b.run()
             b.energies()
@@ -181,7 +188,8 @@ def run_batch(api, form, n, seed):
             dev.close()
     return {"systems": n, "o_range": [4, 20], "v_range": [40, 160],
             "form": form, "items": items, "flop": flop,
-            "prep_s": round(prep, 3), "graph_ms": round(ms, 3),
+            "prep_s": round(prep, 3), "tuned_sizes": timed,
+            "tune_s": round(tune_s, 2), "graph_ms": round(ms, 3),
             "wall_s": round(wall, 4), "tflops": round(flop / ms / 1e9, 3),
             "sequential_plans_s": round(seq, 4),
             "speedup_vs_sequential": round(seq / wall, 3),
@@ -311,7 +319,8 @@ def main():
                     "the others report their energy difference against it")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tune", type=int, default=0,
-                    help="autotune the tile order before timing (1); the "
+                    help="autotune the tile order before timing (1), also "
+                         "of every batch system with --batch; the "
                          "default blocking is the tuner's choice for this "
                          "size to within 1 %%, and leaving the tuner's sample "
                          "launches out keeps the kernel statistics of a "
@@ -456,7 +465,8 @@ def main():
     if args.batch > 0:
         out["batch"] = {}
         for form in forms:
-            out["batch"][form] = run_batch(api, form, args.batch, args.seed)
+            out["batch"][form] = run_batch(api, form, args.batch, args.seed,
+                                           args.tune, args.tune_sample)
             log(f"[bench] batch {form}: {out['batch'][form]}")
     if args.cpu != "none":
         co, cv = (int(x) for x in args.cpu.split(","))

@@ -118,6 +118,13 @@ double libpt_amd_batch_last_ms(libpt_amd_batch *batch);
 /* reference-formulation flops of one run, and its work items */
 double libpt_amd_batch_flop(const libpt_amd_batch *batch);
 int64_t libpt_amd_batch_items(const libpt_amd_batch *batch);
+/* tile order of every system (as libpt_amd_plan_tune): systems whose sizes
+ * have a recorded blocking take it, the others are tuned on the first
+ * `sample_ijk` ijk triples of each spin case (0: 8) and recorded, so each
+ * distinct size is timed once; later runs of the batch use the new orders.
+ * Returns the number of systems timed, -1 on error.  The tuning runs
+ * overwrite the batch's energies until its next run. */
+int libpt_amd_batch_tune(libpt_amd_batch *batch, int sample_ijk);
 
 /* ---- device memory and synthetic inputs ---- */
 void *libpt_amd_dev_alloc(size_t bytes);

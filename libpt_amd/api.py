@@ -117,6 +117,8 @@ def lib():
         L.libpt_amd_batch_flop.argtypes = [_p]
         L.libpt_amd_batch_items.restype = _i64
         L.libpt_amd_batch_items.argtypes = [_p]
+        L.libpt_amd_batch_tune.restype = _i
+        L.libpt_amd_batch_tune.argtypes = [_p, _i]
         L.libpt_amd_dev_alloc.restype = _p
         L.libpt_amd_dev_alloc.argtypes = [_sz]
         L.libpt_amd_dev_free.argtypes = [_p]
@@ -411,6 +413,14 @@ class Batch:
 
     def last_ms(self):
         return lib().libpt_amd_batch_last_ms(self.h)
+
+    def tune(self, sample_ijk=8):
+        """Autotune every system's tile order (libpt_amd_batch_tune);
+        returns the number of systems timed."""
+        r = lib().libpt_amd_batch_tune(self.h, sample_ijk)
+        if r < 0:
+            raise LibptError(last_error())
+        return r
 
     @property
     def flop(self):

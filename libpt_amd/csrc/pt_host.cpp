@@ -1062,10 +1062,10 @@ libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out3)
 /* ---------------------------------------------------------------------
  * Batches: many independent systems in one graph launch.  Each system is a
  * plan (operands staged and packed once, at creation); a segment is one
- * spin case of one system.  The graph holds one batched tile launch over
- * every item of every segment, ordered by cost per item (largest first, so
- * the small items of small systems fill the tail), and one batched
- * reduction that leaves one fixed-order sum per segment.
+ * spin case of one system.  The graph holds the segments' tile launches in
+ * one chain, ordered by cost per item (largest first, so the small items of
+ * small systems fill the tail), and one batched reduction that leaves one
+ * fixed-order sum per segment.
  */
 struct libpt_amd_batch {
 	int call = 0, form = 0, n = 0, nset = 1;
@@ -1265,6 +1265,40 @@ extern "C" int64_t
 libpt_amd_batch_items(const libpt_amd_batch *b)
 {
 	return b ? b->total : 0;
+}
+
+/*
+ * Tile order of every system of a batch: a system whose sizes have a
+ * recorded blocking takes it, the others are tuned on their own plan
+ * (libpt_amd_plan_tune, which records its choice), so each distinct size is
+ * timed once.  The blocking only rewrites a plan's tile list in place, so
+ * the recorded graph reads the new order on its next replay.  Returns the
+ * number of systems timed, or -1.
+ */
+extern "C" int
+libpt_amd_batch_tune(libpt_amd_batch *b, int sample_ijk)
+{
+	if (!dev_init())
+		return -1;
+	if (!b) {
+		set_err("null batch");
+		return -1;
+	}
+	HCHK(hipStreamSynchronize(b->stream), -1);
+	int timed = 0;
+	for (libpt_amd_plan *pl : b->plans) {
+		TileBlock tb;
+		if (tune_lookup(pl->form, pl->call, pl->oa, pl->va, pl->ob, pl->vb,
+		    tb)) {
+			if (libpt_amd_plan_set_block(pl, tb.ga, tb.gb, tb.gc) != 0)
+				return -1;
+		} else {
+			if (libpt_amd_plan_tune(pl, sample_ijk, nullptr) != 0)
+				return -1;
+			++timed;
+		}
+	}
+	return timed;
 }
 
 /* GPU time of the last libpt_amd_batch_run, ms (synchronises) */

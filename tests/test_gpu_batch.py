@@ -125,6 +125,31 @@ def test_batch_split_forms(kind, dims_list, kw, form):
         _close(systems)
 
 
+def test_batch_tune():
+    """Tuned tile orders: each distinct size is timed once (a second tune
+    times none), and the tuned batch's energies equal, bit for bit, those of
+    plans of the same sizes created afterwards (which take the recorded
+    order), and match the oracle."""
+    dims_list = [(7, 41), (5, 26), (7, 41), (4, 19)]
+    systems = _systems("rpt", dims_list, seed0=40, sig_f=1e-2)
+    try:
+        with api.Batch("rpt", "fp64", [(d, dev) for d, _, dev in systems]) as b:
+            timed = b.tune(2)
+            print("systems timed", timed)
+            assert 0 <= timed <= 3
+            assert b.tune(2) == 0
+            b.run()
+            eb = b.energies()
+        for (d, spec, dev), e in zip(systems, eb):
+            ea = _alone("rpt", "fp64", d, dev)
+            eo = oracle.energy("rpt", d, api.synth_host("rpt", spec))
+            print(d, "batch", e, "alone", ea, "oracle", eo)
+            assert e == ea
+            assert abs(e - eo) <= max(1e-10, 1e-13 * abs(eo))
+    finally:
+        _close(systems)
+
+
 def test_batch_graph_replays():
     """Replays of the recorded graph, with other eager work and
     synchronisation between them, all give the first replay's energies."""
