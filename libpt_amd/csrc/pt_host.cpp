@@ -1293,7 +1293,19 @@ libpt_amd_batch_tune(libpt_amd_batch *b, int sample_ijk)
 			if (libpt_amd_plan_set_block(pl, tb.ga, tb.gb, tb.gc) != 0)
 				return -1;
 		} else {
-			if (libpt_amd_plan_tune(pl, sample_ijk, nullptr) != 0)
+			/* (fT): the tuning runs use the batch's save areas
+			 * rather than allocating one set per plan */
+			const bool lend = b->wsave && !pl->wsave;
+			if (lend) {
+				pl->wsave = b->wsave;
+				pl->wslots = b->wslots;
+			}
+			const int rc = libpt_amd_plan_tune(pl, sample_ijk, nullptr);
+			if (lend) {
+				pl->wsave = nullptr;
+				pl->wslots = 0;
+			}
+			if (rc != 0)
 				return -1;
 			++timed;
 		}
