@@ -264,12 +264,20 @@ def cpu_baseline(api, o, v, seed):
     e = fn("rpt", (o, v), arr)
     dt = time.perf_counter() - t0
     f = api.flop("rpt", (o, v))
+    # the same sample through the GPU drop-in (host arrays in), checked
+    # against the CPU energy
+    t0 = time.perf_counter()
+    e_gpu = api.rpt(o, v, *arr)
+    gdt = time.perf_counter() - t0
     return {"value": round(f / dt / 1e12, 4), "unit": "TFLOP/s",
             "cores": threads, "kind": kind, "host": host_info(),
             "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)" if kind ==
             "reference" else "none (restatement)",
             "sample": f"libpt_rpt o={o} v={v} FP64 (full call, "
-                      f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})"}
+                      f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})",
+            "gpu_same_sample": {"E_gpu": e_gpu, "E_cpu": e,
+                                "rel_dE": abs(e_gpu - e) / abs(e),
+                                "wall_s": round(gdt, 3)}}
 
 
 def full_call(form, kind, dims, seed, omega):

@@ -149,6 +149,10 @@ def test_deterministic():
     ("upt", (5, 20, 4, 23), dict(sig_f=1e-2)),
     ("rft", (5, 30), dict(sig_f=1e-2, omega=0.3)),
     ("uft", (4, 18, 3, 21), dict(sig_f=1e-2, omega=0.3)),
+    # sizes above the known-answer files (seconds of reference CPU time)
+    ("rpt", (12, 100), dict(sig_f=1e-2)),
+    ("upt", (9, 70, 8, 74), dict(sig_f=1e-2)),
+    ("rft", (10, 90), dict(sig_f=1e-2, omega=0.3)),
 ])
 def test_fp64_against_compiled_reference(kind, dims, kw):
     """The GPU against libpt itself (oracle/_ref, built from the reference's
