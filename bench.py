@@ -252,10 +252,7 @@ def cpu_baseline(api, o, v, seed):
     sequential mode) on a bounded sample; falls back to the oracle."""
     os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
-    if threads <= 0:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        os.environ["OMP_NUM_THREADS"] = str(threads)
+    threads = ref_threads()
     spec = api.make_spec("rpt", (o, v), seed=seed)
     arr = api.synth_host("rpt", spec)
     kind = "reference" if oracle.ref_available() else "port"
@@ -278,6 +275,53 @@ def cpu_baseline(api, o, v, seed):
             "gpu_same_sample": {"E_gpu": e_gpu, "E_cpu": e,
                                 "rel_dE": abs(e_gpu - e) / abs(e),
                                 "wall_s": round(gdt, 3)}}
+
+
+def ref_threads():
+    """OpenMP threads of the CPU reference: OMP_NUM_THREADS, else the host
+    cores this process may use, at most 16"""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    return threads
+
+
+def ref_check(kind, dims, seed, omega):
+    """One FP64 drop-in call on the GPU and the same call of libpt compiled
+    from its sources (OpenMP on the host cores, MKL sequential) on the same
+    synthetic host arrays: both energies, their difference and both wall
+    times."""
+    from libpt_amd import api
+    import oracle
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    threads = ref_threads()
+    if not oracle.ref_available():
+        print("--ref-check needs oracle/_ref (libpt built from its sources)",
+              file=sys.stderr)
+        sys.exit(2)
+    spec = api.make_spec(kind, dims, seed=seed, omega=omega)
+    dev = api.synth_device(kind, spec)
+    host = [api.from_device(p, n) for p, n in zip(dev.ptrs, dev.sizes)]
+    dev.close()
+    f = api.flop(kind, dims)
+    t0 = time.perf_counter()
+    e_gpu = api.dropin(kind, dims, host)
+    tg = time.perf_counter() - t0
+    log(f"[bench] ref check: GPU libpt_{kind} done in {tg:.2f} s; "
+        f"reference on {threads} threads ({f / 0.4e12:.0f} s at 0.4 TFLOP/s)")
+    t0 = time.perf_counter()
+    e_ref = oracle.ref_energy(kind, dims, host)
+    tr = time.perf_counter() - t0
+    print(json.dumps({"ref_check": {
+        "call": f"libpt_{kind}", "dims": list(dims), "omega": omega,
+        "seed": seed, "flop": f, "E_gpu": e_gpu, "E_ref": e_ref,
+        "abs_dE": abs(e_gpu - e_ref),
+        "rel_dE": abs(e_gpu - e_ref) / abs(e_ref) if e_ref else None,
+        "gpu_wall_s": round(tg, 3), "gpu_tflops": round(f / tg / 1e12, 3),
+        "ref_wall_s": round(tr, 2), "ref_tflops": round(f / tr / 1e12, 4),
+        "ref_cores": threads, "host": host_info(),
+        "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)"}}), flush=True)
 
 
 def full_call(form, kind, dims, seed, omega):
@@ -355,14 +399,24 @@ def main():
                     help="only time one whole host-array call (host to device "
                          "copy included) in this form and exit; fp64 is the "
                          "pt.h drop-in path")
+    ap.add_argument("--ref-check", type=int, default=0,
+                    help="only run one FP64 drop-in call of --call/--dims on "
+                    "the GPU and in libpt compiled from its sources, on the "
+                    "same host arrays, and exit (1)")
     ap.add_argument("--call", default="rpt",
-                    help="--full-call: rpt, upt, rft or uft")
+                    help="--full-call, --ref-check: rpt, upt, rft or uft")
     ap.add_argument("--dims", default="",
                     help="--full-call: o,v (restricted) or oa,va,ob,vb; "
                          "default --o,--v")
     ap.add_argument("--omega", type=float, default=0.3,
                     help="--full-call of rft/uft: excitation energy (Eh)")
     args = ap.parse_args()
+    if args.ref_check:
+        dims = tuple(int(x) for x in args.dims.split(",")) if args.dims \
+            else (args.o, args.v)
+        ref_check(args.call, dims, args.seed,
+                  args.omega if args.call.endswith("ft") else 0.0)
+        return
     if args.full_call:
         dims = tuple(int(x) for x in args.dims.split(",")) if args.dims \
             else (args.o, args.v)
