@@ -174,6 +174,8 @@ def test_fp64_against_compiled_reference(kind, dims, kw):
     ("upt", (5, 20, 4, 23), dict(sig_f=1e-2)),
     ("rft", (5, 30), dict(sig_f=1e-2, omega=0.3)),
     ("uft", (4, 18, 3, 21), dict(sig_f=1e-2, omega=0.3)),
+    ("rpt", (12, 100), dict(sig_f=1e-2)),
+    ("uft", (8, 60, 7, 64), dict(sig_f=1e-2, omega=0.3)),
 ])
 def test_fp32_against_compiled_reference(kind, dims, kw):
     """_mp against libpt's _mp (sgemm, float W, Z and Dn, double sum):
