@@ -324,6 +324,49 @@ def ref_check(kind, dims, seed, omega):
         "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)"}}), flush=True)
 
 
+def ref_check_batch(n, seed):
+    """The n-system restricted batch (o in [4, 20], v in [40, 160]) as one
+    FP64 graph on the GPU, and every system in libpt compiled from its
+    sources on the same arrays: per-system energies and the largest
+    difference."""
+    from libpt_amd import api
+    import oracle
+    os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    threads = ref_threads()
+    if not oracle.ref_available():
+        print("--ref-check needs oracle/_ref (libpt built from its sources)",
+              file=sys.stderr)
+        sys.exit(2)
+    dims = api.batch_dims(n, seed=seed)
+    specs = [api.make_spec("rpt", d, seed=seed + 1 + k)
+             for k, d in enumerate(dims)]
+    systems = [(d, api.synth_device("rpt", s)) for d, s in zip(dims, specs)]
+    try:
+        with api.Batch("rpt", "fp64", systems) as b:
+            b.run()
+            e_gpu = b.energies()
+            ms = b.last_ms()
+    finally:
+        for _, dev in systems:
+            dev.close()
+    rows, tr = [], 0.0
+    for k, (d, s) in enumerate(zip(dims, specs)):
+        t0 = time.perf_counter()
+        e_ref = oracle.ref_energy("rpt", d, api.synth_host("rpt", s))
+        tr += time.perf_counter() - t0
+        rows.append({"dims": list(d), "E_gpu": float(e_gpu[k]),
+                     "E_ref": e_ref, "abs_dE": abs(float(e_gpu[k]) - e_ref)})
+        log(f"[bench] ref check batch: system {k} {d} |dE| "
+            f"{rows[-1]['abs_dE']:.2e}")
+    worst = max(rows, key=lambda r: r["abs_dE"] / max(abs(r["E_ref"]), 1e-300))
+    print(json.dumps({"ref_check_batch": {
+        "systems": n, "seed": seed, "graph_ms": round(ms, 2),
+        "ref_wall_s": round(tr, 2), "ref_cores": threads,
+        "max_abs_dE": max(r["abs_dE"] for r in rows),
+        "max_rel_dE": worst["abs_dE"] / abs(worst["E_ref"]),
+        "worst": worst, "per_system": rows}}), flush=True)
+
+
 def full_call(form, kind, dims, seed, omega):
     """One pt.h-style call with host arrays: the inputs are generated in
     HBM, copied to pageable host memory and freed on the device, then the call
@@ -402,7 +445,8 @@ def main():
     ap.add_argument("--ref-check", type=int, default=0,
                     help="only run one FP64 drop-in call of --call/--dims on "
                     "the GPU and in libpt compiled from its sources, on the "
-                    "same host arrays, and exit (1)")
+                    "same host arrays, and exit (1); with --batch n, the "
+                    "n-system batch instead, every system checked")
     ap.add_argument("--call", default="rpt",
                     help="--full-call, --ref-check: rpt, upt, rft or uft")
     ap.add_argument("--dims", default="",
@@ -411,6 +455,9 @@ def main():
     ap.add_argument("--omega", type=float, default=0.3,
                     help="--full-call of rft/uft: excitation energy (Eh)")
     args = ap.parse_args()
+    if args.ref_check and args.batch > 0:
+        ref_check_batch(args.batch, args.seed)
+        return
     if args.ref_check:
         dims = tuple(int(x) for x in args.dims.split(",")) if args.dims \
             else (args.o, args.v)
