@@ -1,6 +1,6 @@
 """Parity of the HIP engine (through its C ABI) with the CPU oracle and
 libpt's known answers.  Sizes are ones the oracle finishes in seconds;
-size-independent properties cover the full-size runs in bench.py."""
+size-independent properties cover o=45, v=477 (test_gpu_fullsize.py)."""
 
 import numpy as np
 import pytest
