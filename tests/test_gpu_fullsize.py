@@ -1,12 +1,14 @@
-"""rpt at o=45, v=477 (the size of the throughput measurement: 74 GB of FP64
-inputs in HBM, packed operands beside them) through size-independent
-properties on the first ijk triples of each spin case:
+"""The full-size configurations -- rpt o=45 v=477 (the throughput
+measurement: 74 GB of FP64 inputs in HBM, packed operands beside them), upt
+24/200/20/204 and rft 30/300 -- through size-independent properties on the
+first ijk triples of each spin case:
 - repeated runs give bitwise the same per-ijk energies;
 - doubling the denominators (d_ov -> 2 d_ov, exact in binary) halves every
   per-ijk energy bit for bit: W and Z do not depend on d_ov, every element's
   term is divided by exactly twice its denominator, and compensated sums of
   halved terms are the halved sums;
-- split-BF16 agrees with FP64 on the same triples to 1e-6 relative."""
+- a split form agrees with FP64 on the same triples (split-BF16 to 1e-6
+  relative for (T); F16X3 to 1e-4 for (fT), whose signed products cancel)."""
 
 import numpy as np
 import pytest
@@ -15,7 +17,6 @@ from libpt_amd import api
 
 pytestmark = pytest.mark.gpu
 
-DIMS = (45, 477)
 NIJK = 3   # triples per spin case
 
 
@@ -29,33 +30,38 @@ def _slice(plan):
     return out
 
 
-def test_rpt_full_size_properties():
-    spec = api.make_spec("rpt", DIMS, seed=2024)
-    dev = api.synth_device("rpt", spec)
+@pytest.mark.parametrize("kind,dims,omega,split,bound", [
+    ("rpt", (45, 477), 0.0, "bf16x3", 1e-6),
+    ("upt", (24, 200, 20, 204), 0.0, "bf16x3", 1e-6),
+    ("rft", (30, 300), 0.3, "f16x3", 1e-4),
+])
+def test_full_size_properties(kind, dims, omega, split, bound):
+    spec = api.make_spec(kind, dims, seed=2024, omega=omega)
+    dev = api.synth_device(kind, spec)
     d2 = None
     try:
-        with api.Plan("rpt", "fp64", DIMS, dev) as pl:
+        with api.Plan(kind, "fp64", dims, dev) as pl:
             e1 = _slice(pl)
             e1b = _slice(pl)
-        print("fp64", [a.tolist() for a in e1])
+        print(kind, "fp64", [a.tolist() for a in e1])
         for a, b in zip(e1, e1b):
             assert np.array_equal(a, b)
             assert np.all(np.isfinite(a)) and np.all(a != 0)
         # d_ov is the first pt.h array
         dov = api.from_device(dev.ptrs[0], dev.sizes[0])
-        d2 = api.to_device("rpt", [2.0 * dov])
-        dev2 = api.DeviceArrays("rpt", spec, [d2.ptrs[0]] + dev.ptrs[1:],
+        d2 = api.to_device(kind, [2.0 * dov])
+        dev2 = api.DeviceArrays(kind, spec, [d2.ptrs[0]] + dev.ptrs[1:],
                                 dev.sizes, False)
-        with api.Plan("rpt", "fp64", DIMS, dev2) as pl:
+        with api.Plan(kind, "fp64", dims, dev2) as pl:
             e2 = _slice(pl)
         for a, b in zip(e1, e2):
             assert np.array_equal(b, 0.5 * a)
-        with api.Plan("rpt", "bf16x3", DIMS, dev) as pl:
+        with api.Plan(kind, split, dims, dev) as pl:
             es = _slice(pl)
         for a, b in zip(e1, es):
             rel = abs(b.sum() - a.sum()) / abs(a.sum())
-            print("bf16x3 rel", rel)
-            assert rel <= 1e-6
+            print(kind, split, "rel", rel)
+            assert rel <= bound
     finally:
         if d2 is not None:
             d2.close()
