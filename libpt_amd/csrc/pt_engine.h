@@ -21,8 +21,8 @@
 enum pt_form {
 	PT_FORM_FP64 = 0,   /* FP64 operands, v_mfma_f64_16x16x4_f64 */
 	PT_FORM_FP32 = 1,   /* FP32 operands, v_mfma_f32_16x16x4_f32 (_mp) */
-	PT_FORM_BF16X3 = 2, /* hi+lo BF16 pairs, 3 v_mfma_f32_16x16x32_bf16 */
-	PT_FORM_F16X3 = 3,  /* hi+lo FP16 pairs (scaled), 3 x 16x16x32_f16 */
+	PT_FORM_BF16X3 = 2, /* hi+lo BF16 pairs, 3 v_mfma_f32_16x16x16_bf16 */
+	PT_FORM_F16X3 = 3,  /* hi+lo FP16 pairs (scaled), 3 x 16x16x16_f16 */
 	PT_NFORM = 4
 };
 

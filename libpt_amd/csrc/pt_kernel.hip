@@ -22,10 +22,10 @@
  *
  * Forms: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32),
  * split BF16 / FP16 (hi+lo pairs, hi*hi + hi*lo + lo*hi with
- * v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulation, FP64 epilogue).
+ * v_mfma_f32_16x16x16_{bf16,f16}, FP32 accumulation, FP64 epilogue).
  *
  * MFMA lane layouts (checked on gfx950): A[m][k] lane m + 16*(k/KL),
- * B[k][n] lane n + 16*(k/KL) (KL = 1 for x4, 8 consecutive k for x32);
+ * B[k][n] lane n + 16*(k/KL) (KL = 1 for x4, 4 consecutive k for x16);
  * D[m][n] lane n + 16*(m/4) reg m%4 for f32 results, but
  * lane n + 16*(m%4) reg m/4 for v_mfma_f64_16x16x4_f64.
  */
