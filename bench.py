@@ -247,7 +247,7 @@ def run_config(api, name, kind, dims, frac, kw, forms, seed):
     return out
 
 
-def cpu_baseline(api, o, v, seed):
+def cpu_baseline(api, o, v, seed, check=False):
     """The reference libpt compiled from its sources (OpenMP, MKL in
     sequential mode) on a bounded sample; falls back to the oracle."""
     os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
@@ -261,20 +261,23 @@ def cpu_baseline(api, o, v, seed):
     e = fn("rpt", (o, v), arr)
     dt = time.perf_counter() - t0
     f = api.flop("rpt", (o, v))
-    # the same sample through the GPU drop-in (host arrays in), checked
-    # against the CPU energy
-    t0 = time.perf_counter()
-    e_gpu = api.rpt(o, v, *arr)
-    gdt = time.perf_counter() - t0
-    return {"value": round(f / dt / 1e12, 4), "unit": "TFLOP/s",
-            "cores": threads, "kind": kind, "host": host_info(),
-            "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)" if kind ==
-            "reference" else "none (restatement)",
-            "sample": f"libpt_rpt o={o} v={v} FP64 (full call, "
-                      f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})",
-            "gpu_same_sample": {"E_gpu": e_gpu, "E_cpu": e,
-                                "rel_dE": abs(e_gpu - e) / abs(e),
-                                "wall_s": round(gdt, 3)}}
+    out = {"value": round(f / dt / 1e12, 4), "unit": "TFLOP/s",
+           "cores": threads, "kind": kind, "host": host_info(),
+           "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)" if kind ==
+           "reference" else "none (restatement)",
+           "sample": f"libpt_rpt o={o} v={v} FP64 (full call, "
+                     f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})"}
+    if check:
+        # the same sample through the GPU drop-in (host arrays in), checked
+        # against the CPU energy; opt-in, since its launches share the tile
+        # kernels and would mix into a profiled default run's statistics
+        t0 = time.perf_counter()
+        e_gpu = api.rpt(o, v, *arr)
+        gdt = time.perf_counter() - t0
+        out["gpu_same_sample"] = {"E_gpu": e_gpu, "E_cpu": e,
+                                  "rel_dE": abs(e_gpu - e) / abs(e),
+                                  "wall_s": round(gdt, 3)}
+    return out
 
 
 def ref_threads():
@@ -435,6 +438,9 @@ def main():
                     "would mix into a profiled default run's statistics")
     ap.add_argument("--cpu", default="16,160",
                     help="o,v of the CPU-baseline sample; 'none' to skip")
+    ap.add_argument("--cpu-check", type=int, default=0,
+                    help="also run the CPU-baseline sample through the GPU "
+                    "drop-in and report the energy difference (1)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device copy of this many GB (for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -579,7 +585,8 @@ def main():
             log(f"[bench] batch {form}: {out['batch'][form]}")
     if args.cpu != "none":
         co, cv = (int(x) for x in args.cpu.split(","))
-        out["cpu_baseline"] = cpu_baseline(api, co, cv, args.seed)
+        out["cpu_baseline"] = cpu_baseline(api, co, cv, args.seed,
+                                           args.cpu_check)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

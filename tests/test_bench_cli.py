@@ -26,5 +26,6 @@ def test_help_lists_modes():
     r = _bench("--help")
     assert r.returncode == 0
     for opt in ("--forms", "--full-call", "--ref-check", "--batch", "--tune",
+                "--cpu-check",
                 "--copy-calibration", "--cpu"):
         assert opt in r.stdout
