@@ -247,10 +247,23 @@ def run_config(api, name, kind, dims, frac, kw, forms, seed):
     return out
 
 
-def cpu_baseline(api, o, v, seed, check=False):
-    """The reference libpt compiled from its sources (OpenMP, MKL in
-    sequential mode) on a bounded sample; falls back to the oracle."""
+def omp_set_threads(n):
+    """Set the OpenMP thread count of later parallel regions started from
+    this thread (libgomp, shared by the reference library)."""
+    import ctypes
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_baseline(api, o, v, seed, full, check=False):
+    """The reference libpt compiled from its sources (OpenMP threads spread
+    over the cores, MKL in sequential mode) on a bounded sample: the whole
+    o=45 ijk list at a reduced v, extrapolated to the full v by the
+    reference formulation's flop ratio; falls back to the oracle.  Also
+    the reference's own thread spread: one small call (rpt o=8 v=64) on
+    all threads and on one thread."""
     os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
     threads = ref_threads()
     spec = api.make_spec("rpt", (o, v), seed=seed)
@@ -261,12 +274,31 @@ def cpu_baseline(api, o, v, seed, check=False):
     e = fn("rpt", (o, v), arr)
     dt = time.perf_counter() - t0
     f = api.flop("rpt", (o, v))
-    out = {"value": round(f / dt / 1e12, 4), "unit": "TFLOP/s",
+    f_full = api.flop("rpt", full)
+    rate = f / dt
+    out = {"value": round(rate / 1e12, 4), "unit": "TFLOP/s",
            "cores": threads, "kind": kind, "host": host_info(),
            "blas": "MKL (MKL_THREADING_LAYER=SEQUENTIAL)" if kind ==
            "reference" else "none (restatement)",
-           "sample": f"libpt_rpt o={o} v={v} FP64 (full call, "
-                     f"{f:.3e} flop, {dt:.1f} s, E={e:.6e})"}
+           "omp": {k: os.environ.get(k) for k in
+                   ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+           "sample": f"libpt_rpt o={o} v={v} FP64 (full call: every ijk "
+                     f"triple of o={o} at v={v}, {f:.3e} flop, {dt:.1f} s, "
+                     f"E={e:.6e})",
+           "extrapolated_full_s": {
+               "dims": list(full), "s": round(f_full / rate, 1),
+               "how": "flop ratio of the reference formulation "
+                      f"({f_full:.4e} / {f:.4e}) at the sample's rate"}}
+    small = api.synth_host("rpt", api.make_spec("rpt", (8, 64), seed=seed,
+                                                sig_f=1e-2))
+    e_n = fn("rpt", (8, 64), small)
+    if kind == "reference":
+        omp_set_threads(1)
+        e_1 = fn("rpt", (8, 64), small)
+        omp_set_threads(threads)
+        out["thread_spread"] = {
+            "call": "libpt_rpt o=8 v=64", f"E_{threads}_threads": e_n,
+            "E_1_thread": e_1, "abs_dE": abs(e_n - e_1)}
     if check:
         # the same sample through the GPU drop-in (host arrays in), checked
         # against the CPU energy; opt-in, since its launches share the tile
@@ -277,6 +309,33 @@ def cpu_baseline(api, o, v, seed, check=False):
         out["gpu_same_sample"] = {"E_gpu": e_gpu, "E_cpu": e,
                                   "rel_dE": abs(e_gpu - e) / abs(e),
                                   "wall_s": round(gdt, 3)}
+    return out
+
+
+FIXTURES = ["rpt02", "rpt09", "upt01", "upt03", "uft02"]
+
+
+def split_on_fixtures(api, forms):
+    """The split forms on libpt's known-answer inputs (molecular amplitudes
+    and integrals, |E| ~ 1e-3 Eh): |dE| in Eh against each file's golden
+    energy (8 decimals), next to the synthetic relative error."""
+    from libpt_amd.fixtures import read_fixture
+    out = {}
+    for form in forms:
+        rows = {}
+        for name in FIXTURES:
+            fx = read_fixture(os.path.join(ROOT, "tests", "golden",
+                                           name + ".dat.gz"))
+            dims = (fx.oa, fx.va) if fx.kind in ("rpt", "rft") else \
+                (fx.oa, fx.va, fx.ob, fx.vb)
+            e = api.energy(fx.kind, form, dims, fx.args())
+            rows[name] = {"E": e, "E_golden": fx.e_ref,
+                          "abs_dE_Eh": abs(e - fx.e_ref)}
+        out[form] = {"max_abs_dE_Eh": max(r["abs_dE_Eh"]
+                                          for r in rows.values()),
+                     "per_fixture": rows}
+        log(f"[bench] {form} on the known-answer inputs: max |dE| "
+            f"{out[form]['max_abs_dE_Eh']:.2e} Eh")
     return out
 
 
@@ -436,8 +495,13 @@ def main():
                     "against their plans run one after another; off by "
                     "default, since its launches share the tile kernels and "
                     "would mix into a profiled default run's statistics")
-    ap.add_argument("--cpu", default="16,160",
-                    help="o,v of the CPU-baseline sample; 'none' to skip")
+    ap.add_argument("--cpu", default="45,64",
+                    help="o,v of the CPU-baseline sample (every ijk triple "
+                    "of o at a reduced v, extrapolated to --o/--v by the "
+                    "flop ratio); 'none' to skip")
+    ap.add_argument("--fixtures", type=int, default=1,
+                    help="run the split forms of --forms on libpt's five "
+                    "known-answer inputs and report |dE| in Eh (1)")
     ap.add_argument("--cpu-check", type=int, default=0,
                     help="also run the CPU-baseline sample through the GPU "
                     "drop-in and report the energy difference (1)")
@@ -574,6 +638,9 @@ def main():
             "rel_dE": abs(q["e_batches"] - r["e_batches"]) /
                       abs(r["e_batches"]),
         }
+    if args.fixtures:
+        out["split_on_fixtures"] = split_on_fixtures(
+            api, [f for f in forms if f in ("bf16x3", "f16x3")])
     if args.configs:
         out["configs"] = [run_config(api, *c, forms, args.seed)
                           for c in CONFIGS]
@@ -586,7 +653,7 @@ def main():
     if args.cpu != "none":
         co, cv = (int(x) for x in args.cpu.split(","))
         out["cpu_baseline"] = cpu_baseline(api, co, cv, args.seed,
-                                           args.cpu_check)
+                                           (args.o, args.v), args.cpu_check)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 

@@ -138,6 +138,10 @@ int libpt_amd_memcpy(void *dst, const void *src, size_t bytes, int kind);
 int libpt_amd_synth(const void *spec, int tensor, void *dst, int on_device,
     int fp32);
 int64_t libpt_amd_synth_size(const void *spec, int tensor);
+/* elements [begin, begin + count) of the same tensor into host memory
+ * (dst[0] is element begin), e.g. one occupied slab of a large input */
+int libpt_amd_synth_range(const void *spec, int tensor, void *dst,
+    int64_t begin, int64_t count, int fp32);
 
 /* copy n doubles device-to-device with a plain kernel (reference point
  * for hardware traffic counters) */

@@ -128,6 +128,8 @@ def lib():
         L.libpt_amd_synth.argtypes = [C.POINTER(SynthSpec), _i, _p, _i, _i]
         L.libpt_amd_synth_size.restype = _i64
         L.libpt_amd_synth_size.argtypes = [C.POINTER(SynthSpec), _i]
+        L.libpt_amd_synth_range.argtypes = [C.POINTER(SynthSpec), _i, _p,
+                                            _i64, _i64, _i]
         L.libpt_amd_copy_kernel.restype = _i
         L.libpt_amd_copy_kernel.argtypes = [_p, _p, _i64]
         L.libpt_amd_last_error.restype = C.c_char_p
@@ -254,6 +256,21 @@ def synth_host(kind, spec: SynthSpec, fp32=False):
             raise LibptError(last_error())
         out.append(a)
     return out
+
+
+def synth_size(spec: SynthSpec, tensor: int) -> int:
+    return int(lib().libpt_amd_synth_size(C.byref(spec), tensor))
+
+
+def synth_range(spec: SynthSpec, tensor: int, out: np.ndarray,
+                begin: int) -> None:
+    """Elements [begin, begin + out.size) of one synthetic tensor into the
+    contiguous host array out (float64 or float32)."""
+    assert out.flags.c_contiguous and out.dtype in (np.float64, np.float32)
+    if lib().libpt_amd_synth_range(C.byref(spec), tensor, out.ctypes.data,
+                                   begin, out.size,
+                                   int(out.dtype == np.float32)) != 0:
+        raise LibptError(last_error())
 
 
 def synth_device(kind, spec: SynthSpec, fp32=False) -> DeviceArrays:

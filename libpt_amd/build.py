@@ -6,12 +6,9 @@
 Every HIP source is compiled with ``--offload-arch=gfx950`` and with packed
 FP32 instructions disabled (``-target-feature -packed-fp32-ops``); the
 host-only sources go through the same driver so they see the HIP headers.
-The kernel source is also built with ``-amdgpu-mfma-vgpr-form``: MFMA
-accumulators then live in ordinary VGPRs, which keeps the tile kernels'
-loop-carried accumulators from being copied into AGPRs on every step.  The
-FP64 two-W-set kernels (256 accumulator registers) and the split-form
-kernels come from ``pt_kernel_agpr.hip``, the same source built without
-that option.
+No hidden code-generation options: the tile kernels keep their MFMA
+accumulators in ordinary VGPRs (no accumulator copies) with the default
+code generation.
 """
 
 from __future__ import annotations
@@ -26,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libpt_amd.so")
 
-SOURCES = ["pt_kernel.hip", "pt_kernel_agpr.hip", "pt_host.cpp", "pt_terms.cpp"]
+SOURCES = ["pt_kernel.hip", "pt_host.cpp", "pt_terms.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -35,14 +32,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Wall", "-I" + CSRC, "-I" + os.path.join(ROOT, "include")]
 
 
-KERNEL_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form"]
-
-
 def _compile(src: str, extra=()) -> subprocess.Popen:
     os.makedirs(OBJ, exist_ok=True)
     out = os.path.join(OBJ, src + ".o")
-    if src == "pt_kernel.hip":
-        extra = [*KERNEL_FLAGS, *extra]
     cmd = [HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", out]
     return subprocess.Popen(cmd, stdout=subprocess.PIPE,
                             stderr=subprocess.STDOUT, text=True)

@@ -16,13 +16,16 @@
 #ifndef LIBPT_AMD_PT_ENGINE_H
 #define LIBPT_AMD_PT_ENGINE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 enum pt_form {
 	PT_FORM_FP64 = 0,   /* FP64 operands, v_mfma_f64_16x16x4_f64 */
 	PT_FORM_FP32 = 1,   /* FP32 operands, v_mfma_f32_16x16x4_f32 (_mp) */
-	PT_FORM_BF16X3 = 2, /* hi+lo BF16 pairs, 3 v_mfma_f32_16x16x16_bf16 */
-	PT_FORM_F16X3 = 3,  /* hi+lo FP16 pairs (scaled), 3 x 16x16x16_f16 */
+	PT_FORM_BF16X3 = 2, /* hi+lo BF16 pairs, 3 products per kappa packed
+	                       into v_mfma_f32_16x16x32_bf16 */
+	PT_FORM_F16X3 = 3,  /* hi+lo FP16 pairs (scaled), the same with
+	                       v_mfma_f32_16x16x32_f16 */
 	PT_NFORM = 4
 };
 
@@ -57,21 +60,35 @@ pt_kgroups(int form, int64_t K)
 	return 2 * ((K + 2 * kc - 1) / (2 * kc));
 }
 
-/* workgroup tile: 32 a x 32 b x TCW c, four waves of 16 x 16 x TCW.
- * TCW = 32 for FP32: its 32 accumulator blocks (4 registers each) and the
- * operands of a step fit the 256 VGPRs.  FP64 blocks take 8 registers and
- * the split forms keep hi / lo operand pairs, so those use 16.  (fT) forms
- * its two W sets one after the other in the same registers, so the width
- * does not depend on the W-set count. */
+/* Workgroup geometry (the kernel's Geo): NW = 4 waves, tile 32 a x 32 b x
+ * TC c with TC = 32 for FP32 and 16 for FP64 and the split forms, two
+ * workgroups per CU; or NW = 8 waves, tile 32 a x 32 b x 32 c, one
+ * workgroup per CU with four staging slots.  Each wave owns 16 a x 16 b x
+ * CW c (CW = TC for four waves, 16 for eight).  (fT) forms its two W sets
+ * one after the other in the same registers, so the geometry does not
+ * depend on the W-set count. */
 #define PT_TA 32
 #define PT_TB 32
-#define PT_THREADS 256
+
+/* default wave count of a form; LIBPT_AMD_WAVES=4 or 8 overrides it for
+ * new plans */
+static inline int
+pt_nw_default(int form)
+{
+	return form == 1 ? 4 : 8;
+}
 
 static inline int
-pt_tcw(int form, int nset)
+pt_tcw(int form, int nset, int nw)
 {
 	(void)nset;
-	return form == 1 ? 32 : 16;
+	return nw == 8 ? 32 : (form == 1 ? 32 : 16);
+}
+
+static inline int
+pt_cw(int form, int nset, int nw)
+{
+	return nw == 8 ? 16 : pt_tcw(form, nset, nw);
 }
 
 /*
@@ -114,7 +131,7 @@ struct pt_launch {
 	int32_t nterm_a;             /* per W set: terms with role a */
 	const int32_t *ijk;          /* [n_ijk][3] */
 	const int32_t *tiles;        /* [n_tiles][3]: blocks (A, B, C), A >= B */
-	int32_t n_tiles, pad_;
+	int32_t n_tiles, nw;         /* tiles per ijk, waves per workgroup */
 	int64_t item0, n_chunk;      /* items [item0, item0 + n_chunk) */
 	double *partial;             /* energy of each item */
 	double inv_scale[2];         /* per W set: 1 / (operand scale product) */
@@ -129,26 +146,25 @@ struct pt_launch {
 #ifdef __cplusplus
 extern "C" {
 #endif
-/* workgroups per CU the tile kernels are built for */
-#define PT_WG_PER_CU 2
-
-/* items per launch with two W sets: each item of a launch has its own save
- * area, so this bounds the area (4 GB for FP64) */
-#define PT_FT_CHUNK 32768
-
-/* bytes of one item's saved W set: 4 waves x TCW blocks x 4 values x 64
+/* bytes of one item's saved W set: NW waves x CW blocks x 4 values x 64
  * lanes */
 static inline int64_t
-pt_wsave_bytes(int form, int nset)
+pt_wsave_bytes(int form, int nset, int nw)
 {
-	return (int64_t)4 * pt_tcw(form, nset) * 4 * 64 * (form == 0 ? 8 : 4);
+	return (int64_t)nw * pt_cw(form, nset, nw) * 4 * 64 *
+	    (form == 0 ? 8 : 4);
+}
+
+/* items per launch with two W sets: each item of a launch has its own save
+ * area, so this bounds the areas to 4 GB (32768 items of 128 KB) */
+static inline int64_t
+pt_ft_chunk(int form, int nw)
+{
+	return ((int64_t)4 << 30) / pt_wsave_bytes(form, 2, nw);
 }
 
 /* kernel launchers (pt_kernel.hip); return hipError_t as int */
 int pt_launch_tiles(int form, const struct pt_launch *p, void *stream);
-/* the kernels built without VGPR-form MFMA (pt_kernel_agpr.hip): FP64
- * with two W sets and the split forms */
-int pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream);
 /* out[s] = fixed-order sum of segment s's partials (as pt_launch_reduce) */
 int pt_launch_reduce_batch(const struct pt_launch *segs,
     const int64_t *begin, int nseg, double *out, void *stream);
@@ -165,6 +181,7 @@ int pt_launch_absmax(const double *src, int64_t n, double *out, void *stream);
 int pt_launch_copy(const double *src, double *dst, int64_t n, void *stream);
 int pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n,
     int fp32, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

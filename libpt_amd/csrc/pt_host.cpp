@@ -19,8 +19,10 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/libpt_amd.h"
@@ -222,12 +224,15 @@ tune_load()
 	fclose(f);
 }
 
+/* may_load: read LIBPT_AMD_TUNE_CACHE first if not yet read (the
+ * allocation-free entry point only looks at entries already in memory) */
 bool
 tune_lookup(int form, int call, int64_t oa, int64_t va, int64_t ob,
-    int64_t vb, TileBlock &tb)
+    int64_t vb, TileBlock &tb, bool may_load)
 {
 	std::lock_guard<std::mutex> lk(g_mu);
-	tune_load();
+	if (may_load)
+		tune_load();
 	/* the latest entry wins */
 	for (size_t n = g_tune.size(); n-- > 0;) {
 		const TuneEntry &t = g_tune[n];
@@ -261,7 +266,7 @@ struct Case {
 	TileBlock block;
 	int same = 0, nA = 0, nC = 0, tcw = 16;  /* tile-list geometry */
 	pt_launch L;
-	std::vector<pt_term> terms;
+	pt_term_list terms;
 	pt_term *d_terms = nullptr;
 	int32_t *d_ijk = nullptr, *d_tiles = nullptr;
 	double *d_partial = nullptr;
@@ -277,13 +282,25 @@ struct Blocks {
 
 } /* namespace */
 
+/* device memory of an allocation-free call: the plan takes its buffers
+ * from [base, base + cap) in order, 256-byte aligned */
+struct Arena {
+	char *base = nullptr;
+	size_t cap = 0, used = 0;
+};
+
 struct libpt_amd_plan {
 	int call = 0, form = 0;
+	int nw = 4;                 /* waves per workgroup of the tile kernel */
 	int64_t oa = 0, va = 0, ob = 0, vb = 0;
 	hipStream_t stream = nullptr;
+	bool own_stream = true;     /* created (and destroyed) by the plan */
 	hipEvent_t ev0 = nullptr, ev1 = nullptr;
-	std::vector<Case> cases;
-	std::vector<void *> owned;
+	FixedVec<Case, 4> cases;
+	/* device buffers from hipMalloc, freed with the plan; with an arena
+	 * every buffer comes from it and nothing is allocated or freed */
+	FixedVec<void *, 64> owned;
+	Arena *arena = nullptr;
 	double *d_out = nullptr;
 	double scale[4] = {1, 1, 1, 1};
 	double prep_s = 0;
@@ -295,29 +312,32 @@ struct libpt_amd_plan {
 
 namespace {
 
+/* waits for the plan's work, then frees what it owns (not the plan) */
+void
+plan_release(libpt_amd_plan *pl)
+{
+	/* a caller's stream may be the null stream */
+	if (pl->stream || !pl->own_stream)
+		(void)hipStreamSynchronize(pl->stream);
+	for (void *p : pl->owned)
+		(void)hipFree(p);
+	pl->owned.clear();
+	if (pl->ev0)
+		(void)hipEventDestroy(pl->ev0);
+	if (pl->ev1)
+		(void)hipEventDestroy(pl->ev1);
+	if (pl->stream && pl->own_stream)
+		(void)hipStreamDestroy(pl->stream);
+	pl->ev0 = pl->ev1 = nullptr;
+	pl->stream = nullptr;
+}
+
 void
 plan_free(libpt_amd_plan *pl)
 {
 	if (!pl)
 		return;
-	if (pl->stream)
-		(void)hipStreamSynchronize(pl->stream);
-	for (Case &c : pl->cases) {
-		(void)hipFree(c.d_terms);
-		(void)hipFree(c.d_ijk);
-		(void)hipFree(c.d_tiles);
-		(void)hipFree(c.d_partial);
-	}
-	for (void *p : pl->owned)
-		(void)hipFree(p);
-	(void)hipFree(pl->wsave);
-	(void)hipFree(pl->d_out);
-	if (pl->ev0)
-		(void)hipEventDestroy(pl->ev0);
-	if (pl->ev1)
-		(void)hipEventDestroy(pl->ev1);
-	if (pl->stream)
-		(void)hipStreamDestroy(pl->stream);
+	plan_release(pl);
 	delete pl;
 }
 
@@ -333,13 +353,97 @@ dev_alloc(void **p, size_t bytes)
 	return true;
 }
 
+/* a device buffer of the plan: from its arena, else from hipMalloc
+ * (freed with the plan) */
+bool
+plan_alloc(libpt_amd_plan *pl, void **p, size_t bytes)
+{
+	*p = nullptr;
+	if (pl->arena) {
+		Arena &a = *pl->arena;
+		const size_t need = (bytes + 255) & ~(size_t)255;
+		if (need > a.cap - a.used) {
+			set_err("the call needs more than the %zu bytes of static "
+			    "scratch of the allocation-free entry point (%zu in use, "
+			    "%zu more requested)", a.cap, a.used, need);
+			return false;
+		}
+		*p = a.base + a.used;
+		a.used += need;
+		return true;
+	}
+	if (pl->owned.size() >= 64) {
+		set_err("too many device buffers in one plan");
+		return false;
+	}
+	if (!dev_alloc(p, bytes))
+		return false;
+	pl->owned.push_back(*p);
+	return true;
+}
+
+/* Host lists go to the device in chunks of N triples of ints through one
+ * chunk buffer: host scratch from the libpt_set_malloc hooks (as the
+ * reference takes its scratch, src/pt.c:208,223), or a stack array for
+ * the allocation-free entry point. */
+enum { LIST_CHUNK = 2048 };
+
+struct ListBuf {
+	int32_t stack_[3 * LIST_CHUNK];
+	HostBuf<int32_t> heap_;
+	int32_t *p = nullptr;
+	bool get(const libpt_amd_plan *pl)
+	{
+		if (pl->arena) {
+			p = stack_;
+			return true;
+		}
+		if (!heap_.alloc(3 * LIST_CHUNK))
+			return false;
+		p = heap_.p;
+		return true;
+	}
+};
+
+struct ListUpload {
+	int32_t *buf;
+	int n = 0;
+	int64_t done = 0;
+	int32_t *dst;
+	bool ok = true;
+	ListUpload(int32_t *d, int32_t *b) : buf(b), dst(d) {}
+	void put(int a, int b, int c)
+	{
+		buf[3 * n] = a;
+		buf[3 * n + 1] = b;
+		buf[3 * n + 2] = c;
+		if (++n == LIST_CHUNK)
+			flush();
+	}
+	bool flush()
+	{
+		if (n > 0 && ok) {
+			hipError_t e = hipMemcpy(dst + 3 * done, buf,
+			    3 * sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+			if (e != hipSuccess) {
+				set_err("list upload failed: %s", hipGetErrorString(e));
+				ok = false;
+			}
+		}
+		done += n;
+		n = 0;
+		return ok;
+	}
+};
+
 /* Tiles in blocks of GA x GB x GC (A, B, C): the workgroups that run at the
  * same time on one XCD take consecutive items, and a compact block lets
  * them share the Q slices of all three roles in L2 (Q[B][C] across A,
  * Q[A][C] across B, Q[A][B] across C).  Writes 3 ints per tile; returns
  * the tile count (the same for every blocking). */
+template <class Emit>
 int
-tile_order(const Case &c, const TileBlock &tb, int32_t *out)
+tile_order(const Case &c, const TileBlock &tb, Emit emit)
 {
 	const int nA = c.nA, nC = c.nC, tcw = c.tcw;
 	int nt = 0;
@@ -352,9 +456,7 @@ tile_order(const Case &c, const TileBlock &tb, int32_t *out)
 			/* same spin needs some b > c: c_min < b_max */
 			if (c.same && Cb * tcw >= B * PT_TB + PT_TB - 1)
 				continue;
-			out[3 * nt] = A;
-			out[3 * nt + 1] = B;
-			out[3 * nt + 2] = Cb;
+			emit(A, B, Cb);
 			++nt;
 		}
 	return nt;
@@ -362,19 +464,23 @@ tile_order(const Case &c, const TileBlock &tb, int32_t *out)
 
 /* a new blocking for an existing case (same tiles, new order) */
 bool
-case_set_block(Case &c, const TileBlock &tb)
+case_set_block(const libpt_amd_plan *pl, Case &c, const TileBlock &tb)
 {
 	if (c.n_tiles == 0)
 		return true;
-	HostBuf<int32_t> tiles;
-	if (!tiles.alloc(3 * (size_t)c.n_tiles))
+	ListBuf lb;
+	if (!lb.get(pl))
 		return false;
-	if (tile_order(c, tb, tiles.p) != c.n_tiles) {
+	int nt = 0;
+	(void)tile_order(c, tb, [&](int, int, int) { ++nt; });
+	if (nt != c.n_tiles) {
 		set_err("tile order changed the tile count");
 		return false;
 	}
-	HCHK(hipMemcpy(c.d_tiles, tiles.p, 3 * sizeof(int32_t) * (size_t)c.n_tiles,
-	    hipMemcpyHostToDevice), false);
+	ListUpload up(c.d_tiles, lb.p);
+	(void)tile_order(c, tb, [&](int a, int b, int x) { up.put(a, b, x); });
+	if (!up.flush())
+		return false;
 	c.block = tb;
 	return true;
 }
@@ -383,51 +489,50 @@ case_set_block(Case &c, const TileBlock &tb)
  * the same-spin case (a > b > c) only c tiles that reach below the top of
  * the B block. */
 bool
-case_lists(Case &c, int same, int64_t o1, int64_t o2, int64_t na, int64_t nc,
-    int tcw)
+case_lists(libpt_amd_plan *pl, Case &c, int same, int64_t o1, int64_t o2,
+    int64_t na, int64_t nc, int tcw)
 {
 	int64_t nijk = same ? o1 * (o1 - 1) * (o1 - 2) / 6 : o1 * (o1 - 1) / 2 * o2;
-	HostBuf<int32_t> ijk;
-	if (!ijk.alloc(3 * (size_t)(nijk > 0 ? nijk : 1)))
-		return false;
-	int64_t it = 0;
-	for (int64_t i = 0; i < o1; ++i)
-		for (int64_t j = i + 1; j < o1; ++j)
-			for (int64_t k = same ? j + 1 : 0; k < (same ? o1 : o2); ++k) {
-				ijk.p[3 * it] = (int32_t)i;
-				ijk.p[3 * it + 1] = (int32_t)j;
-				ijk.p[3 * it + 2] = (int32_t)k;
-				++it;
-			}
 	c.same = same;
 	c.nA = (int)((na + PT_TA - 1) / PT_TA);
 	c.nC = (int)((nc + tcw - 1) / tcw);
 	c.tcw = tcw;
-	HostBuf<int32_t> tiles;
-	if (!tiles.alloc(3 * (size_t)c.nA * (c.nA + 1) / 2 * c.nC))
-		return false;
-	const int nt = tile_order(c, c.block, tiles.p);
+	int nt = 0;
+	(void)tile_order(c, c.block, [&](int, int, int) { ++nt; });
 	c.n_ijk = nijk;
 	c.n_tiles = nt;
-	if (!dev_alloc((void **)&c.d_ijk, 3 * sizeof(int32_t) * (size_t)nijk) ||
-	    !dev_alloc((void **)&c.d_tiles, 3 * sizeof(int32_t) * (size_t)nt) ||
-	    !dev_alloc((void **)&c.d_partial,
+	if (!plan_alloc(pl, (void **)&c.d_ijk, 3 * sizeof(int32_t) *
+	    (size_t)nijk) ||
+	    !plan_alloc(pl, (void **)&c.d_tiles, 3 * sizeof(int32_t) *
+	    (size_t)nt) ||
+	    !plan_alloc(pl, (void **)&c.d_partial,
 	    sizeof(double) * (size_t)(nijk * nt)))
 		return false;
-	HCHK(hipMemcpy(c.d_ijk, ijk.p, 3 * sizeof(int32_t) * (size_t)nijk,
-	    hipMemcpyHostToDevice), false);
-	HCHK(hipMemcpy(c.d_tiles, tiles.p, 3 * sizeof(int32_t) * (size_t)nt,
-	    hipMemcpyHostToDevice), false);
+	ListBuf lb;
+	if (!lb.get(pl))
+		return false;
+	ListUpload ui(c.d_ijk, lb.p);
+	for (int64_t i = 0; i < o1; ++i)
+		for (int64_t j = i + 1; j < o1; ++j)
+			for (int64_t k = same ? j + 1 : 0; k < (same ? o1 : o2); ++k)
+				ui.put((int)i, (int)j, (int)k);
+	if (!ui.flush())
+		return false;
+	ListUpload ut(c.d_tiles, lb.p);
+	(void)tile_order(c, c.block, [&](int a, int b, int x) { ut.put(a, b, x); });
+	if (!ut.flush())
+		return false;
 	HCHK(hipMemset(c.d_partial, 0, sizeof(double) * (size_t)(nijk * nt)),
 	    false);
 	return true;
 }
 
 bool
-case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
-    int64_t ob, int64_t vb)
+case_finish(libpt_amd_plan *pl, Case &c, int form, int kind, int nset,
+    int64_t oa, int64_t va, int64_t ob, int64_t vb)
 {
 	pt_launch &L = c.L;
+	L.nw = pl->nw;
 	L.kind = kind;
 	L.nset = nset;
 	L.oa = oa;
@@ -436,7 +541,7 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 	L.vb = vb;
 	L.na = (int32_t)va;
 	L.nc = (int32_t)vb;
-	L.tcw = pt_tcw(form, nset);
+	L.tcw = pt_tcw(form, nset, pl->nw);
 	for (int s = 0; s < nset; ++s) {
 		if (kind == PT_AAA)
 			pt_terms_same(c.terms, s, oa, va);
@@ -454,12 +559,14 @@ case_finish(Case &c, int form, int kind, int nset, int64_t oa, int64_t va,
 	}
 	L.nterm1 = pt_terms_order(c.terms, &L.nterm_a);
 	L.nterm2 = (int32_t)c.terms.size() / nset - L.nterm1;
-	if (!dev_alloc((void **)&c.d_terms, sizeof(pt_term) * c.terms.size()))
+	if (!plan_alloc(pl, (void **)&c.d_terms, sizeof(pt_term) *
+	    (size_t)c.terms.size()))
 		return false;
 	HCHK(hipMemcpy(c.d_terms, c.terms.data(),
-	    sizeof(pt_term) * c.terms.size(), hipMemcpyHostToDevice), false);
+	    sizeof(pt_term) * (size_t)c.terms.size(), hipMemcpyHostToDevice),
+	    false);
 	L.terms = c.d_terms;
-	if (!case_lists(c, kind == PT_AAA, oa, ob, va, vb, L.tcw))
+	if (!case_lists(pl, c, kind == PT_AAA, oa, ob, va, vb, L.tcw))
 		return false;
 	L.ijk = c.d_ijk;
 	L.tiles = c.d_tiles;
@@ -495,7 +602,7 @@ family_scale(libpt_amd_plan *pl, const double *const *arr, const int64_t *n,
 namespace {
 
 /* two W sets: the save areas of one launch's items, sized to the largest
- * spin case (at most PT_FT_CHUNK); no-op when present */
+ * spin case (at most pt_ft_chunk); no-op when present */
 bool
 plan_wsave(libpt_amd_plan *pl)
 {
@@ -504,32 +611,41 @@ plan_wsave(libpt_amd_plan *pl)
 	int64_t most = 1;
 	for (const Case &d : pl->cases)
 		most = std::max(most, d.n_ijk * d.n_tiles);
-	const int64_t slots = std::min<int64_t>(PT_FT_CHUNK, most);
-	if (!dev_alloc(&pl->wsave, (size_t)slots *
-	    (size_t)pt_wsave_bytes(pl->form, 2)))
+	int64_t slots = std::min<int64_t>(pt_ft_chunk(pl->form, pl->nw), most);
+	const int64_t wb = pt_wsave_bytes(pl->form, 2, pl->nw);
+	if (pl->arena) {
+		/* what the scratch has left, in whole areas: launches then hold
+		 * fewer items */
+		const int64_t left = (int64_t)(pl->arena->cap - pl->arena->used) / wb;
+		slots = std::min(slots, left > 0 ? left : 1);
+	}
+	if (!plan_alloc(pl, &pl->wsave, (size_t)slots * (size_t)wb))
 		return false;
 	pl->wslots = slots;
 	return true;
 }
 
-/* own_wsave: allocate the (fT) save areas now, so that runs allocate
- * nothing (plans owned by a batch use the batch's) */
-libpt_amd_plan *
-plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
-    size_t vb_, const void *const *arr, bool own_wsave)
+/* Stages the operands and work lists of one call into pl, which comes
+ * with its arena (or none) and, when pl->own_stream is false, its stream.
+ * own_wsave: allocate the (fT) save areas now, so that runs allocate
+ * nothing (plans owned by a batch use the batch's).  On failure the
+ * caller releases pl. */
+bool
+plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
+    size_t ob_, size_t vb_, const void *const *arr, bool own_wsave)
 {
-	if (!dev_init())
-		return nullptr;
 	if (call < 0 || call > 3 || form < 0 || form >= PT_NFORM) {
 		set_err("bad call %d or form %d", call, form);
-		return nullptr;
+		return false;
 	}
 	const bool ft = call == LIBPT_AMD_RFT || call == LIBPT_AMD_UFT;
 	const bool restricted = call == LIBPT_AMD_RPT || call == LIBPT_AMD_RFT;
 	int64_t oa = (int64_t)oa_, va = (int64_t)va_;
 	int64_t ob = restricted ? oa : (int64_t)ob_;
 	int64_t vb = restricted ? va : (int64_t)vb_;
-	libpt_amd_plan *pl = new libpt_amd_plan;
+	pl->nw = pt_nw_default(form);
+	if (const char *w = getenv("LIBPT_AMD_WAVES"))
+		pl->nw = atoi(w) == 4 ? 4 : atoi(w) == 8 ? 8 : pl->nw;
 	pl->call = call;
 	pl->form = form;
 	pl->oa = oa;
@@ -538,16 +654,17 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	pl->vb = vb;
 #define PFAIL(cond)                  \
 	do {                         \
-		if (!(cond)) {       \
-			plan_free(pl); \
-			return nullptr; \
-		}                    \
+		if (!(cond))         \
+			return false; \
 	} while (0)
-	PFAIL(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) ==
-	    hipSuccess || (set_err("hipStreamCreate failed"), false));
-	PFAIL(hipEventCreate(&pl->ev0) == hipSuccess &&
-	    hipEventCreate(&pl->ev1) == hipSuccess);
-	PFAIL(dev_alloc((void **)&pl->d_out, 4 * sizeof(double)));
+	if (pl->own_stream) {
+		PFAIL(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) ==
+		    hipSuccess || (set_err("hipStreamCreate failed"), false));
+		PFAIL((hipEventCreate(&pl->ev0) == hipSuccess &&
+		    hipEventCreate(&pl->ev1) == hipSuccess) ||
+		    (set_err("hipEventCreate failed"), false));
+	}
+	PFAIL(plan_alloc(pl, (void **)&pl->d_out, 4 * sizeof(double)));
 
 	int64_t n[10];
 	const int narr = api_sizes(call, oa, va, ob, vb, n);
@@ -622,8 +739,8 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 				continue;
 			}
 			void *dst;
-			PFAIL(dev_alloc(&dst, sizeof(uint32_t) * (size_t)n[fam2[q]]));
-			pl->owned.push_back(dst);
+			PFAIL(plan_alloc(pl, &dst, sizeof(uint32_t) *
+			    (size_t)n[fam2[q]]));
 			PFAIL(pt_launch_split((const double *)arr[fam2[q]],
 			    (uint32_t *)dst, n[fam2[q]], form == PT_FORM_F16X3, sc2[q],
 			    pl->stream) == 0 || (set_err("split launch failed"), false));
@@ -643,8 +760,7 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 				const Geo g = geo(fam, b);
 				const int64_t nkb = pt_kgroups(form, g.K);
 				void *dst;
-				PFAIL(dev_alloc(&dst, (size_t)(g.R * nkb * g.PL) * GB));
-				pl->owned.push_back(dst);
+				PFAIL(plan_alloc(pl, &dst, (size_t)(g.R * nkb * g.PL) * GB));
 				const char *src = (const char *)arr[api] + esz * (size_t)g.off;
 				PFAIL(pt_launch_pack(src, dst, g.R, g.K, nkb, g.PL, KC, form,
 				    sc, pl->stream) == 0 ||
@@ -725,7 +841,7 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	 * cache, else the default */
 	TileBlock blk;
 	if (!tile_block_env(blk))
-		(void)tune_lookup(form, call, oa, va, ob, vb, blk);
+		(void)tune_lookup(form, call, oa, va, ob, vb, blk, !pl->arena);
 	for (const CaseSpec &cs : specs) {
 		Case c;
 		c.block = blk;
@@ -769,11 +885,13 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 		if (cs.kind == PT_AAA) {
 			empty = o1 < 3 || v1 < 3;
 			if (!empty)
-				PFAIL(case_finish(c, form, PT_AAA, nset, o1, v1, o1, v1));
+				PFAIL(case_finish(pl, c, form, PT_AAA, nset, o1, v1, o1,
+				    v1));
 		} else {
 			empty = o1 < 2 || v1 < 2 || o2 < 1 || v2 < 1;
 			if (!empty)
-				PFAIL(case_finish(c, form, PT_AAB, nset, o1, v1, o2, v2));
+				PFAIL(case_finish(pl, c, form, PT_AAB, nset, o1, v1, o2,
+				    v2));
 		}
 		pl->cases.push_back(c);
 	}
@@ -784,6 +902,20 @@ plan_create(int call, int form, size_t oa_, size_t va_, size_t ob_,
 	PFAIL(hipDeviceSynchronize() == hipSuccess ||
 	    (set_err("hipDeviceSynchronize failed"), false));
 #undef PFAIL
+	return true;
+}
+
+libpt_amd_plan *
+plan_create(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
+    const void *const *arr, bool own_wsave)
+{
+	if (!dev_init())
+		return nullptr;
+	libpt_amd_plan *pl = new libpt_amd_plan;
+	if (!plan_init(pl, call, form, oa, va, ob, vb, arr, own_wsave)) {
+		plan_free(pl);
+		return nullptr;
+	}
 	return pl;
 }
 
@@ -990,7 +1122,7 @@ libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
 	tb.gc = gc;
 	HCHK(hipStreamSynchronize(pl->stream), -1);
 	for (Case &c : pl->cases)
-		if (!case_set_block(c, tb))
+		if (!case_set_block(pl, c, tb))
 			return -1;
 	return 0;
 }
@@ -1143,8 +1275,9 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	} while (0)
 	BFAIL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) ==
 	    hipSuccess || (set_err("hipStreamCreate failed"), false));
-	BFAIL(hipEventCreate(&b->ev0) == hipSuccess &&
-	    hipEventCreate(&b->ev1) == hipSuccess);
+	BFAIL((hipEventCreate(&b->ev0) == hipSuccess &&
+	    hipEventCreate(&b->ev1) == hipSuccess) ||
+	    (set_err("hipEventCreate failed"), false));
 
 	struct Seg {
 		pt_launch L;
@@ -1196,12 +1329,13 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(dev_alloc((void **)&b->d_segs, sizeof(pt_launch) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_begin, sizeof(int64_t) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_out, sizeof(double) * (nseg + 1)));
-	const int64_t wbytes = pt_wsave_bytes(form, 2);
+	const int nw = b->plans.empty() ? 4 : b->plans[0]->nw;
+	const int64_t wbytes = pt_wsave_bytes(form, 2, nw);
 	if (b->nset == 2 && b->total > 0) {
 		int64_t most = 1;
 		for (const Seg &g : segs)
 			most = std::max(most, g.items);
-		b->wslots = std::min<int64_t>(PT_FT_CHUNK, most);
+		b->wslots = std::min<int64_t>(pt_ft_chunk(form, nw), most);
 		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots * (size_t)wbytes));
 	}
 	if (nseg > 0) {
@@ -1289,7 +1423,7 @@ libpt_amd_batch_tune(libpt_amd_batch *b, int sample_ijk)
 	for (libpt_amd_plan *pl : b->plans) {
 		TileBlock tb;
 		if (tune_lookup(pl->form, pl->call, pl->oa, pl->va, pl->ob, pl->vb,
-		    tb)) {
+		    tb, true)) {
 			if (libpt_amd_plan_set_block(pl, tb.ga, tb.gb, tb.gc) != 0)
 				return -1;
 		} else {
@@ -1429,6 +1563,27 @@ libpt_amd_synth(const void *spec_, int tensor, void *dst, int on_device,
 }
 
 extern "C" int
+libpt_amd_synth_range(const void *spec_, int tensor, void *dst, int64_t begin,
+    int64_t count, int fp32)
+{
+	const pt_synth_spec *spec = (const pt_synth_spec *)spec_;
+	const int64_t n = pt_synth_size(spec, tensor);
+	if (begin < 0 || count < 0 || begin > n || count > n - begin) {
+		set_err("synth range [%lld, +%lld) outside tensor of %lld",
+		    (long long)begin, (long long)count, (long long)n);
+		return -1;
+	}
+	for (int64_t t = 0; t < count; ++t) {
+		double x = pt_synth_elem(spec, tensor, begin + t);
+		if (fp32)
+			((float *)dst)[t] = (float)x;
+		else
+			((double *)dst)[t] = x;
+	}
+	return 0;
+}
+
+extern "C" int
 libpt_amd_copy_kernel(void *dst, const void *src, int64_t n)
 {
 	if (!dev_init())
@@ -1448,6 +1603,107 @@ libpt_amd_last_error(void)
 }
 
 namespace {
+
+/*
+ * Host-to-device copy of the caller's pageable arrays.  Small inputs go
+ * with hipMemcpy.  Large ones are cut into chunks of H2D_CHUNK bytes that
+ * H2D_WORKERS host threads take in turn: a worker copies its chunk into
+ * one of its two pinned buffers and hands it to its own stream, so that
+ * the host copies of several threads and the PCIe transfers overlap.
+ */
+enum { H2D_WORKERS = 8 };
+constexpr size_t H2D_CHUNK = (size_t)32 << 20;
+constexpr size_t H2D_PINNED_FROM = (size_t)256 << 20;
+
+bool
+h2d_copy(void *const *dst, const void *const *src, const size_t *bytes,
+    int n)
+{
+	size_t total = 0;
+	for (int q = 0; q < n; ++q)
+		total += bytes[q];
+	if (total < H2D_PINNED_FROM) {
+		for (int q = 0; q < n; ++q)
+			if (bytes[q] > 0)
+				HCHK(hipMemcpy(dst[q], src[q], bytes[q],
+				    hipMemcpyHostToDevice), false);
+		return true;
+	}
+	/* chunk list: (array, offset) in array order */
+	std::vector<std::pair<int, size_t>> chunks;
+	for (int q = 0; q < n; ++q)
+		for (size_t o = 0; o < bytes[q]; o += H2D_CHUNK)
+			chunks.push_back({q, o});
+	const unsigned hw = std::thread::hardware_concurrency();
+	const int nw = (int)std::max(1u, std::min<unsigned>(H2D_WORKERS,
+	    hw > 2 ? hw / 2 : 1));
+	std::atomic<size_t> next(0);
+	std::atomic<bool> failed(false);
+	int dev = 0;
+	HCHK(hipGetDevice(&dev), false);
+	std::mutex emu;
+	std::string emsg;
+	auto fail = [&](const char *what, hipError_t e) {
+		std::lock_guard<std::mutex> lk(emu);
+		if (!failed.exchange(true))
+			emsg = std::string(what) + ": " + hipGetErrorString(e);
+	};
+	auto worker = [&]() {
+		hipStream_t st = nullptr;
+		void *buf[2] = {nullptr, nullptr};
+		hipEvent_t ev[2] = {nullptr, nullptr};
+		hipError_t e = hipSetDevice(dev);
+		if (e == hipSuccess)
+			e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+		for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+			e = hipHostMalloc(&buf[b], H2D_CHUNK, hipHostMallocDefault);
+			if (e == hipSuccess)
+				e = hipEventCreateWithFlags(&ev[b],
+				    hipEventDisableTiming);
+		}
+		if (e != hipSuccess)
+			fail("staging set-up failed", e);
+		for (size_t k = 0, use = 0; e == hipSuccess && !failed; ++use) {
+			k = next.fetch_add(1);
+			if (k >= chunks.size())
+				break;
+			const int b = (int)(use & 1);
+			const int q = chunks[k].first;
+			const size_t o = chunks[k].second;
+			const size_t len = std::min(H2D_CHUNK, bytes[q] - o);
+			/* the buffer's previous transfer must be done */
+			if ((e = hipEventSynchronize(ev[b])) != hipSuccess)
+				break;
+			memcpy(buf[b], (const char *)src[q] + o, len);
+			e = hipMemcpyAsync((char *)dst[q] + o, buf[b], len,
+			    hipMemcpyHostToDevice, st);
+			if (e == hipSuccess)
+				e = hipEventRecord(ev[b], st);
+		}
+		if (e != hipSuccess)
+			fail("host to device copy failed", e);
+		if (st)
+			(void)hipStreamSynchronize(st);
+		for (int b = 0; b < 2; ++b) {
+			if (ev[b])
+				(void)hipEventDestroy(ev[b]);
+			if (buf[b])
+				(void)hipHostFree(buf[b]);
+		}
+		if (st)
+			(void)hipStreamDestroy(st);
+	};
+	std::vector<std::thread> th;
+	for (int w = 0; w < nw; ++w)
+		th.emplace_back(worker);
+	for (std::thread &t : th)
+		t.join();
+	if (failed) {
+		set_err("%s", emsg.c_str());
+		return false;
+	}
+	return true;
+}
 
 /* Host arrays -> HBM -> plan -> energy.  Returns NaN on failure. */
 double
@@ -1471,18 +1727,13 @@ host_call(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
 	libpt_amd_plan *pl = nullptr;
 	double t1 = 0, t2 = 0;
 	bool ok = true;
+	size_t nbytes[10];
 	for (int q = 0; q < narr && ok; ++q) {
-		ok = dev_alloc(&d[q], el * (size_t)n[q]);
-		if (ok && n[q] > 0) {
-			hipError_t he = hipMemcpy(d[q], arrays[q], el * (size_t)n[q],
-			    hipMemcpyHostToDevice);
-			if (he != hipSuccess) {
-				set_err("host to device copy failed: %s",
-				    hipGetErrorString(he));
-				ok = false;
-			}
-		}
+		nbytes[q] = el * (size_t)n[q];
+		ok = dev_alloc(&d[q], nbytes[q]);
 	}
+	if (ok)
+		ok = h2d_copy(d, arrays, nbytes, narr);
 	t1 = now_s();
 	if (ok)
 		pl = libpt_amd_plan_create(call, form, oa, va, ob, vb, d);
@@ -1536,6 +1787,7 @@ double
 dropin(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
     const void *const *arrays)
 {
+	g_err[0] = 0;
 	double e = host_call(call, form, oa, va, ob, vb, arrays, nullptr);
 	if (isnan(e) && g_err[0]) {
 		fprintf(stderr, "libpt: %s\n", g_err);

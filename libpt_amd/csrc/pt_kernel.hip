@@ -21,8 +21,9 @@
  * contiguous global index mapped to consecutive threads.
  *
  * Forms: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32),
- * split BF16 / FP16 (hi+lo pairs, hi*hi + hi*lo + lo*hi with
- * v_mfma_f32_16x16x16_{bf16,f16}, FP32 accumulation, FP64 epilogue).
+ * split BF16 / FP16 (hi+lo pairs, hi*hi + hi*lo + lo*hi packed into the
+ * k slots of v_mfma_f32_16x16x32_{bf16,f16}, FP32 accumulation, FP64
+ * epilogue).
  *
  * MFMA lane layouts (checked on gfx950): A[m][k] lane m + 16*(k/KL),
  * B[k][n] lane n + 16*(k/KL) (KL = 1 for x4, 4 consecutive k for x16);
@@ -37,16 +38,13 @@
 #include "pt_engine.h"
 #include "synth.h"
 
-#ifndef PT_KERNEL_AGPR
-#define PT_KERNEL_AGPR 0
-#endif
-
+typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef _Float16 v4h __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 namespace {
 
@@ -98,44 +96,45 @@ mma(float a, float b, v4f c)
 	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-/* one K=16 bf16 / f16 MFMA on operands given as two 32-bit words each
- * (halves k0, k1 and k2, k3 of the lane's kappa) */
+/* one K=32 bf16 / f16 MFMA (v_mfma_f32_16x16x32_{bf16,f16}, full rate on
+ * gfx950) on operands of four 32-bit words (eight 16-bit values) per lane.
+ * A and B fill their eight slots of each lane group with the same
+ * (kappa, hi/lo) pattern, so each slot contributes one product of the
+ * same kappa whatever the instruction's k order within the lane group. */
 template <int F>
 __device__ __forceinline__ v4f
-mma16(uint32_t a01, uint32_t a23, uint32_t b01, uint32_t b23, v4f c)
+mma32(v4u a, v4u b, v4f c)
 {
-	const v2u a = {a01, a23}, b = {b01, b23};
 	if constexpr (F == PT_FORM_BF16X3)
-		return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-		    __builtin_bit_cast(v4s, a), __builtin_bit_cast(v4s, b), c, 0, 0,
-		    0);
+		return __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+		    __builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
+		    0, 0);
 	else
-		return __builtin_amdgcn_mfma_f32_16x16x16f16(
-		    __builtin_bit_cast(v4h, a), __builtin_bit_cast(v4h, b), c, 0, 0,
+		return __builtin_amdgcn_mfma_f32_16x16x32_f16(
+		    __builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0,
 		    0);
 }
 
-/* hi*hi + hi*lo + lo*hi (FP32 accumulation) of one split operand pair.
+/* hi*hi + lo*hi + hi*lo (FP32 accumulation) of one split operand pair.
  * Operand words (LDS layout of the split forms): w0 = hi k0 k1,
- * w1 = lo k0 k1, w2 = hi k2 k3, w3 = lo k2 k3; the small products first. */
-template <int F>
+ * w1 = lo k0 k1, w2 = hi k2 k3, w3 = lo k2 k3 (type A: four kappa per
+ * lane group) or w0, w1 only (type B: two kappa).
+ *   type A: slots (hi k0-3, lo k0-3) x (hi k0-3, hi k0-3) give hi*hi and
+ *           lo*hi; slots (hi k0-3, 0) x (lo k0-3, 0) give hi*lo, first;
+ *   type B: slots (hi k0 k1, lo k0 k1, hi k0 k1, 0 0) x
+ *           (hi k0 k1, hi k0 k1, lo k0 k1, 0 0) give all three at once. */
+template <int F, bool TA>
 __device__ __forceinline__ v4f
 mma_split(v4u a, v4u b, v4f c)
 {
-	c = mma16<F>(a[1], a[3], b[0], b[2], c);
-	c = mma16<F>(a[0], a[2], b[1], b[3], c);
-	c = mma16<F>(a[0], a[2], b[0], b[2], c);
-	return c;
-}
-
-template <class G>
-__device__ __forceinline__ G
-neg_if(G v, int neg)
-{
-	if constexpr (sizeof(G) == 4 && !__is_same(G, float))
-		return neg ? (v ^ 0x80008000u) : v; /* both halves */
-	else
-		return neg ? -v : v;
+	if constexpr (TA) {
+		c = mma32<F>(v4u{a[0], a[2], 0u, 0u}, v4u{b[1], b[3], 0u, 0u}, c);
+		return mma32<F>(v4u{a[0], a[2], a[1], a[3]},
+		    v4u{b[0], b[2], b[0], b[2]}, c);
+	} else {
+		return mma32<F>(v4u{a[0], a[1], a[0], 0u},
+		    v4u{b[0], b[0], b[1], 0u}, c);
+	}
 }
 
 /* ---------------------------------------------------------- epilogue */
@@ -218,7 +217,7 @@ struct Step {
 	int64_t sbytes;      /* S rows [0, sext) from sb0 */
 	int64_t qgs;         /* bytes per kappa group of the packed slab */
 	int64_t s_st, q_u, q_w;
-	int role, set, K, neg, packed, ufast;
+	int role, set, K, packed, ufast;
 	int s0, sn, sext, u0, uext, w0, wext, qwe, qsh;
 	int lo_mask, hi_sh;  /* pair index split: fast index bits, shift */
 };
@@ -234,7 +233,6 @@ make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
 	s.role = uni(T.role);
 	s.set = uni(T.set);
 	s.K = uni(T.K);
-	s.neg = uni(T.sign) < 0;
 	s.packed = uni(T.packed);
 	s.ufast = uni(T.ufast);
 	s.s_st = uni64(T.s_st);
@@ -278,21 +276,9 @@ pair_uw(const Step &s, int pidx, int &u, int &w)
 	w = s.ufast ? hi : lo;
 }
 
-/* raw buffer loads: offsets at or beyond the descriptor's byte count
- * return zero, which handles tile edges, invalid pairs and the kappa tail
- * without branches */
-template <class G>
-__device__ __forceinline__ G
-bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
-{
-	if constexpr (sizeof(G) == 8)
-		return __builtin_bit_cast(G,
-		    __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-	else
-		return __builtin_bit_cast(G,
-		    __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-
+/* buffer descriptors: loads at offsets at or beyond the descriptor's byte
+ * count return zero, which handles tile edges, invalid pairs and the kappa
+ * tail without branches */
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t
 brsrc(const void *base, int64_t bytes)
 {
@@ -334,8 +320,8 @@ bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
 
 
 /*
- * FP64 / FP32 staging: NSLOT slots, each holding one step (term, kappa
- * chunk) of the tile, loaded NSLOT - 1 steps ahead of the step computed.
+ * Staging: NSLOT slots, each holding one step (term, kappa chunk) of the
+ * tile, loaded NSLOT - 1 steps ahead of the step computed.
  * Slot (bytes):
  *   S: 32 rows of 64 B of data (the chunk's kappa of one row) + padding;
  *   Q: 32 u-rows of 1 KB of data + padding.  A u-row is the chunk of the
@@ -355,13 +341,14 @@ bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
  * modulo 64 bytes (type A: S 96, Q 1056), 8-byte reads 16 modulo 64 (type
  * B: S 80, Q 1040); the other strides cost a 2-way conflict.
  */
+template <int NW, int NS>
 struct Stg {
 	static constexpr int S_ROW_A = 96, S_ROW_B = 80;
 	static constexpr int Q_USTR_A = 1024 + 32, Q_USTR_B = 1024 + 16;
-	static constexpr int S_BYTES = 32 * S_ROW_A;
+	static constexpr int S_BYTES = NW == 8 ? 4096 : 3072;
 	static constexpr int Q_BYTES = 32 * Q_USTR_A;
 	static constexpr int SLOT = S_BYTES + Q_BYTES;
-	static constexpr int NSLOT = 2;
+	static constexpr int NSLOT = NS;
 	static constexpr int BYTES = NSLOT * SLOT;
 	template <bool TA> static constexpr int s_row() {
 		return TA ? S_ROW_A : S_ROW_B;
@@ -371,21 +358,20 @@ struct Stg {
 	}
 };
 
-#define PT_QRUN_W 8   /* DMA runs per wave per step (32 u-rows / 4 waves) */
-
-/* lane offsets of the Q runs of one term (type A: w = lane / 4, 16-byte
- * chunk c = lane % 4 from group c / 2; type B: w = lane / 2, chunk
+/* lane offsets of the Q runs of one term: the 32 u-rows of a step are
+ * dealt to the NW waves, QR = 32 / NW runs each (type A: w = lane / 4,
+ * 16-byte chunk c = lane % 4 from group c / 2; type B: w = lane / 2, chunk
  * lane % 2); gstride = bytes per packed group of the slab */
-template <bool TA>
+template <bool TA, int NW>
 __device__ __forceinline__ void
-qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W])
+qtab(const Step &s, int wave, int lane, uint32_t *vo)
 {
 	const int w = TA ? (lane >> 2) : (lane >> 1);
 	const uint32_t cofs = TA ? (uint32_t)((lane & 1) * 16 +
 	    ((lane >> 1) & 1) * s.qgs) : (uint32_t)((lane & 1) * 16);
 #pragma unroll
-	for (int n = 0; n < PT_QRUN_W; ++n) {
-		const int ug = s.u0 + wave + 4 * n, wg = s.w0 + w;
+	for (int n = 0; n < 32 / NW; ++n) {
+		const int ug = s.u0 + wave + NW * n, wg = s.w0 + w;
 		const bool v = ug < s.uext && wg < s.wext && (!s.packed || ug > wg);
 		const int64_t pidx = s.packed ? (int64_t)ug * (ug - 1) / 2 + wg :
 		    (int64_t)ug * s.q_u + (int64_t)wg * s.q_w;
@@ -394,19 +380,18 @@ qtab(const Step &s, int wave, int lane, uint32_t (&vo)[PT_QRUN_W])
 }
 
 /* DMA of step g (kappa groups g or 2g, 2g + 1) of a term into a slot */
-template <bool TA>
+template <bool TA, int NW>
 __device__ __forceinline__ void
-issue_q(const Step &s, int g, char *sQ, int wave,
-    const uint32_t (&vo)[PT_QRUN_W])
+issue_q(const Step &s, int g, char *sQ, int wave, const uint32_t *vo)
 {
 	constexpr int NG = TA ? 2 : 1;
 	const __amdgpu_buffer_rsrc_t rq = brsrc(s.qb0 + (int64_t)g * NG * s.qgs,
 	    NG * s.qgs);
 #pragma unroll
-	for (int n = 0; n < PT_QRUN_W; ++n)
+	for (int n = 0; n < 32 / NW; ++n)
 		__builtin_amdgcn_raw_ptr_buffer_load_lds(rq,
 		    (__attribute__((address_space(3))) void *)(sQ +
-		    (wave + 4 * n) * Stg::q_ustr<TA>()), 16, vo[n], 0, 0, 0);
+		    (wave + NW * n) * Stg<4, 2>::q_ustr<TA>()), 16, vo[n], 0, 0, 0);
 }
 
 /* wait until at most n vector-memory loads of this wave are outstanding */
@@ -417,219 +402,382 @@ wait_vm()
 	__asm__ volatile("s_waitcnt vmcnt(%0)" :: "n"(N) : "memory");
 }
 
-/* S loads: thread -> (row, kappa k0 .. k0 + SEPT - 1) of a chunk of KS
- * kappa, KS / SEPT threads per row (8 for type A, 4 for type B); rows
- * beyond the tile load zeros and are not stored.  Per term: the byte
- * offset from the chunk base (or PT_OOR) and the LDS store offset (or -1). */
-template <int F, int SEPT>
+/*
+ * S rows by DMA as well: the S region of a slot holds 32 rows at stride
+ * S_ROW_A (96) or S_ROW_B (80) bytes, the step's chunk of kappa (64 or 32
+ * bytes, in the API element type) at the start of each row.  The region is
+ * covered by NW x SD dword DMA instructions per step: dword p = 64 (SD wave
+ * + d) + lane of the region is row p / (stride / 4), dword p % (stride / 4)
+ * of it; padding dwords and rows outside the tile fetch an offset every
+ * descriptor rejects (zero).  The term's sign is applied to the
+ * accumulators (an exact negation before and after its steps), not to S.
+ * Kappa beyond the term's extent read the next S elements, which meet
+ * the zero padding of the packed Q operand.
+ */
+template <int NW>
+struct SDma {
+	static constexpr int SD = NW == 8 ? 2 : 3;   /* instructions per wave */
+	static constexpr int BYTES = NW * SD * 256;  /* S region of a slot */
+};
+
+template <int F, int NW>
 __device__ __forceinline__ void
-s_lane(const Step &s, int tid, bool ta, uint32_t &ofs, int &lofs)
+stab(const Step &s, bool ta, int wave, int lane, uint32_t *so)
 {
 	constexpr int ES = (int)sizeof(typename Form<F>::G);
-	const int tpr = ta ? 8 : 4;
-	const int row = tid / tpr, k0 = (tid % tpr) * SEPT;
-	const int sg = s.s0 + row;
-	ofs = (row < s.sn && sg < s.sext) ?
-	    (uint32_t)(((int64_t)sg * s.s_st + k0) * ES) : PT_OOR;
-	lofs = row < 32 ? row * (ta ? Stg::S_ROW_A : Stg::S_ROW_B) + k0 * ES :
-	    -1;
+	constexpr int SD = SDma<NW>::SD;
+	const int rd = (ta ? Stg<NW, 2>::S_ROW_A : Stg<NW, 2>::S_ROW_B) / 4;
+	const int dd = (ta ? Form<F>::KC : Form<F>::KC / 2) * ES / 4;
+#pragma unroll
+	for (int d = 0; d < SD; ++d) {
+		const int p = 64 * (SD * wave + d) + lane;
+		const int r = p / rd, q = p - r * rd, sg = s.s0 + r;
+		so[d] = (q < dd && r < s.sn && sg < s.sext) ?
+		    (uint32_t)((int64_t)sg * s.s_st * ES + 4 * q) : PT_OOR;
+	}
 }
 
-template <int F, int SEPT>
+/* DMA of the S rows of the step at kappa kc into a slot's S region */
+template <int F, int NW>
 __device__ __forceinline__ void
-load_s(const Step &s, uint32_t ofs, int k0, int kc, int ks,
-    typename Form<F>::G (&sr)[SEPT])
+issue_s(const Step &s, int kc, char *sS, int wave, const uint32_t *so)
 {
-	typedef typename Form<F>::G G;
-	constexpr int ES = (int)sizeof(G);
+	constexpr int ES = (int)sizeof(typename Form<F>::G);
+	constexpr int SD = SDma<NW>::SD;
 	const __amdgpu_buffer_rsrc_t rs = brsrc(s.sb0 + (int64_t)kc * ES,
 	    s.sbytes - (int64_t)kc * ES);
 #pragma unroll
-	for (int e = 0; e < SEPT; ++e) {
-		const uint32_t o = (k0 + e < ks && kc + k0 + e < s.K &&
-		    ofs != PT_OOR) ? ofs + e * ES : PT_OOR;
-		sr[e] = neg_if(bload<G>(rs, o), s.neg);
-	}
+	for (int d = 0; d < SD; ++d)
+		__builtin_amdgcn_raw_ptr_buffer_load_lds(rs,
+		    (__attribute__((address_space(3))) void *)(sS +
+		    256 * (SD * wave + d)), 4, so[d], 0, 0, 0);
 }
 
-template <int F, int SEPT>
-__device__ __forceinline__ void
-store_s(char *sS, int lofs, const typename Form<F>::G (&sr)[SEPT])
+/* a split-form S operand: S rows hold API elements (hi | lo << 16 per
+ * kappa); the MFMA operand words are hi k0 k1, lo k0 k1 (, hi k2 k3,
+ * lo k2 k3), the layout of the packed Q groups */
+__device__ __forceinline__ uint32_t
+hi_pair(uint32_t e0, uint32_t e1)
 {
-	typedef typename Form<F>::G G;
-	if (lofs < 0)
-		return;
-	if constexpr (Form<F>::SPLIT) {
-		/* two kappa (hi | lo << 16 each) -> hi pair word, lo pair word */
-		static_assert(SEPT == 2, "split S pairs");
-		const uint32_t h = (sr[0] & 0xffffu) | (sr[1] << 16);
-		const uint32_t l = (sr[0] >> 16) | (sr[1] & 0xffff0000u);
-		*(v2u *)(sS + lofs) = v2u{h, l};
-	} else {
-#pragma unroll
-		for (int e = 0; e < SEPT; ++e)
-			*(G *)(sS + lofs + e * (int)sizeof(G)) = sr[e];
-	}
+	return (e0 & 0xffffu) | (e1 << 16);
 }
 
-/* a split-form operand from LDS: 16 bytes (type A: hi k0 k1, lo k0 k1,
- * hi k2 k3, lo k2 k3) or 8 bytes (type B: hi k0 k1, lo k0 k1, with k2 and
- * k3 zero in both operands) */
+__device__ __forceinline__ uint32_t
+lo_pair(uint32_t e0, uint32_t e1)
+{
+	return (e0 >> 16) | (e1 & 0xffff0000u);
+}
+
+/* raw words (type A: four elements, type B: two, then zeros) -> the
+ * operand words */
 template <bool TA>
 __device__ __forceinline__ v4u
-split_rd(const char *p)
+split_words_s(v4u e)
 {
 	if constexpr (TA)
-		return *(const v4u *)p;
-	else {
-		const v2u t = *(const v2u *)p;
-		return v4u{t[0], t[1], 0u, 0u};
+		return v4u{hi_pair(e[0], e[1]), lo_pair(e[0], e[1]),
+		    hi_pair(e[2], e[3]), lo_pair(e[2], e[3])};
+	else
+		return v4u{hi_pair(e[0], e[1]), lo_pair(e[0], e[1]), 0u, 0u};
+}
+
+/*
+ * Operand reads of one step from LDS, all of a wave's reads in one asm
+ * block that ends by waiting for them (so the registers it returns hold
+ * their data).  Reading in asm keeps the compiler from placing a wait for
+ * the DMAs in flight (into other slots) before these reads: the order of
+ * DMAs and reads is kept by the explicit waits and barriers of the step
+ * loop.  Type-B rows (32 B per w): S is 8 bytes at pf, Q value x is 8
+ * bytes at pv + 32 x, returned in pairs (x = 2i in words 0-1, 2i + 1 in
+ * words 2-3 of q[i]).  Type-A rows (64 B per w): 16 bytes at pf and at
+ * pv + 64 x.
+ */
+#define PT_A(x) (unsigned)(size_t)(x)
+#define PT_R2(i, o0, o1) "ds_read2_b64 %[q" #i "], %[pv] offset0:" #o0 \
+	" offset1:" #o1 "\n\t"
+
+__device__ __forceinline__ void
+rd_b16(const char *pf, const char *pv, v2u &f, v4u (&q)[8])
+{
+	asm volatile("ds_read_b64 %[f], %[pf]\n\t"
+	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
+	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
+	    "s_waitcnt lgkmcnt(0)"
+	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
+	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
+	      [q6] "=v"(q[6]), [q7] "=v"(q[7])
+	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
+}
+
+/* two S values (pf, pf + 1280: rows 16 apart) and 16 Q values */
+__device__ __forceinline__ void
+rd_b16s2(const char *pf, const char *pv, v2u &f0, v2u &f1, v4u (&q)[8])
+{
+	asm volatile("ds_read_b64 %[f0], %[pf]\n\t"
+	    "ds_read_b64 %[f1], %[pf] offset:1280\n\t"
+	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
+	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
+	    "s_waitcnt lgkmcnt(0)"
+	    : [f0] "=v"(f0), [f1] "=v"(f1), [q0] "=v"(q[0]), [q1] "=v"(q[1]),
+	      [q2] "=v"(q[2]), [q3] "=v"(q[3]), [q4] "=v"(q[4]),
+	      [q5] "=v"(q[5]), [q6] "=v"(q[6]), [q7] "=v"(q[7])
+	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
+}
+
+__device__ __forceinline__ void
+rd_b32(const char *pf, const char *pv, v2u &f, v4u (&q)[16])
+{
+	asm volatile("ds_read_b64 %[f], %[pf]\n\t"
+	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
+	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
+	    PT_R2(8, 64, 68) PT_R2(9, 72, 76) PT_R2(10, 80, 84) PT_R2(11, 88, 92)
+	    PT_R2(12, 96, 100) PT_R2(13, 104, 108) PT_R2(14, 112, 116) PT_R2(15, 120, 124)
+	    "s_waitcnt lgkmcnt(0)"
+	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
+	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
+	      [q6] "=v"(q[6]), [q7] "=v"(q[7]), [q8] "=v"(q[8]),
+	      [q9] "=v"(q[9]), [q10] "=v"(q[10]), [q11] "=v"(q[11]),
+	      [q12] "=v"(q[12]), [q13] "=v"(q[13]), [q14] "=v"(q[14]),
+	      [q15] "=v"(q[15])
+	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
+}
+
+#define PT_R4(i, o) "ds_read_b128 %[q" #i "], %[pv] offset:" #o "\n\t"
+
+__device__ __forceinline__ void
+rd_a16(const char *pf, const char *pv, v4u &f, v4u (&q)[16])
+{
+	asm volatile("ds_read_b128 %[f], %[pf]\n\t"
+	    PT_R4(0, 0) PT_R4(1, 64) PT_R4(2, 128) PT_R4(3, 192) PT_R4(4, 256) PT_R4(5, 320) PT_R4(6, 384)
+	    PT_R4(7, 448) PT_R4(8, 512) PT_R4(9, 576) PT_R4(10, 640) PT_R4(11, 704) PT_R4(12, 768)
+	    PT_R4(13, 832) PT_R4(14, 896) PT_R4(15, 960)
+	    "s_waitcnt lgkmcnt(0)"
+	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
+	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
+	      [q6] "=v"(q[6]), [q7] "=v"(q[7]), [q8] "=v"(q[8]),
+	      [q9] "=v"(q[9]), [q10] "=v"(q[10]), [q11] "=v"(q[11]),
+	      [q12] "=v"(q[12]), [q13] "=v"(q[13]), [q14] "=v"(q[14]),
+	      [q15] "=v"(q[15])
+	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
+}
+
+/* keep everything before it (the operand reads and the DMA issue) ahead
+ * of everything after it (the MFMAs), and the MFMAs ahead of the wait */
+__device__ __forceinline__ void
+sched_fence()
+{
+	__builtin_amdgcn_sched_barrier(0);
+}
+
+/* operand x of a type-B read pair array: 8 bytes as 2 words */
+__device__ __forceinline__ v2u
+pair_of(const v4u *q, int x)
+{
+	const v4u t = q[x >> 1];
+	return (x & 1) ? v2u{t[2], t[3]} : v2u{t[0], t[1]};
+}
+
+/* MFMAs of one type-B operand pair (8 bytes each) into c; SA: the first
+ * operand is the S value (role a), else the second */
+template <int F>
+__device__ __forceinline__ typename Form<F>::ACC
+mma_b(v2u a, v2u b, typename Form<F>::ACC c)
+{
+	if constexpr (F == PT_FORM_FP64) {
+		return mma(__builtin_bit_cast(double, a),
+		    __builtin_bit_cast(double, b), c);
+	} else if constexpr (F == PT_FORM_FP32) {
+		c = mma(__builtin_bit_cast(float, a[0]),
+		    __builtin_bit_cast(float, b[0]), c);
+		return mma(__builtin_bit_cast(float, a[1]),
+		    __builtin_bit_cast(float, b[1]), c);
+	} else {
+		return mma_split<F, false>(v4u{a[0], a[1], 0u, 0u},
+		    v4u{b[0], b[1], 0u, 0u}, c);
 	}
 }
 
-/* an MFMA operand chunk of NB bytes (16: type A, 8: type B) */
-template <class L, int NB>
-struct Chunk {
-	typedef L V __attribute__((ext_vector_type(NB / sizeof(L))));
-	static constexpr int N = NB / sizeof(L);
-};
+/* the same for type-A operands (16 bytes each) */
+template <int F>
+__device__ __forceinline__ typename Form<F>::ACC
+mma_a(v4u a, v4u b, typename Form<F>::ACC c)
+{
+	if constexpr (F == PT_FORM_FP64) {
+		const v2d x = __builtin_bit_cast(v2d, a), y = __builtin_bit_cast(v2d,
+		    b);
+		c = mma(x[0], y[0], c);
+		return mma(x[1], y[1], c);
+	} else if constexpr (F == PT_FORM_FP32) {
+		const v4f x = __builtin_bit_cast(v4f, a), y = __builtin_bit_cast(v4f,
+		    b);
+#pragma unroll
+		for (int e = 0; e < 4; ++e)
+			c = mma(x[e], y[e], c);
+		return c;
+	} else {
+		return mma_split<F, true>(a, b, c);
+	}
+}
+
+/* an S value for the MFMA: the split forms regroup the halves of the API
+ * elements into hi and lo pair words */
+template <int F, bool TA>
+__device__ __forceinline__ v4u
+s_operand(v4u raw)
+{
+	if constexpr (Form<F>::SPLIT)
+		return split_words_s<TA>(raw);
+	else
+		return raw;
+}
 
 /*
  * Phase 1 (accumulator block x = c offset, D[a][b]) over one staged step:
- *   role a (RA): A = S[a] (fixed), B = Q[b][c = x];
- *   role b:      A = Q[a][c = x],  B = S[b] (fixed).
- * TA: type-A rows (64 B per w), else type-B rows (32 B per w).
+ *   role a (RA): A = S[a] (fixed), B = Q[b][c = wc + x];
+ *   role b:      A = Q[a][c = wc + x],  B = S[b] (fixed).
+ * TA: type-A rows (64 B per w), else type-B rows (32 B per w); wc: the
+ * wave's first c within the tile (type B only).  The step's operands are
+ * read first; mid() (the DMAs of a later step) is issued after the reads
+ * and before the MFMAs.
  */
-template <int F, int TCW, bool RA, bool TA>
+template <int F, int TCW, bool RA, bool TA, class Mid>
 __device__ __forceinline__ void
 mma_p1(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
-    int wa, int wb, int ln, int lg)
+    int wa, int wb, int wc, int ln, int lg, Mid mid)
 {
-	typedef typename Form<F>::L L;
 	constexpr int NB = TA ? 16 : 8, WB = TA ? 64 : 32;
 	const int ws = RA ? wa : wb, wu = RA ? wb : wa;
-	const char *pf = sS + (16 * ws + ln) * Stg::s_row<TA>() + NB * lg;
-	const char *pv = sQ + (16 * wu + ln) * Stg::q_ustr<TA>() + NB * lg;
-	if constexpr (Form<F>::SPLIT) {
-		const v4u f = split_rd<TA>(pf);
+	const char *pf = sS + (16 * ws + ln) * Stg<4, 2>::s_row<TA>() + NB * lg;
+	const char *pv = sQ + (16 * wu + ln) * Stg<4, 2>::q_ustr<TA>() + NB * lg +
+	    WB * wc;
+	if constexpr (TA) {
+		static_assert(TCW == 16, "type-A steps have 16 w");
+		v4u fr, q[16];
+		rd_a16(pf, pv, fr, q);
+		mid();
+		sched_fence();
+		const v4u f = s_operand<F, true>(fr);
 #pragma unroll
-		for (int x = 0; x < TCW; ++x) {
-			const v4u q = split_rd<TA>(pv + WB * x);
-			ac[x] = RA ? mma_split<F>(f, q, ac[x]) :
-			    mma_split<F>(q, f, ac[x]);
-		}
+		for (int x = 0; x < 16; ++x)
+			ac[x] = RA ? mma_a<F>(f, q[x], ac[x]) : mma_a<F>(q[x], f, ac[x]);
 	} else {
-		typedef typename Chunk<L, NB>::V V;
-		constexpr int NE = Chunk<L, NB>::N;
-		const V f = *(const V *)pf;
+		v2u fr;
+		v4u q[TCW / 2];
+		if constexpr (TCW == 16)
+			rd_b16(pf, pv, fr, q);
+		else
+			rd_b32(pf, pv, fr, q);
+		mid();
+		sched_fence();
+		const v4u fw = s_operand<F, false>(v4u{fr[0], fr[1], 0u, 0u});
+		const v2u f = v2u{fw[0], fw[1]};
 #pragma unroll
-		for (int x = 0; x < TCW; ++x) {
-			const V q = *(const V *)(pv + WB * x);
-#pragma unroll
-			for (int e = 0; e < NE; ++e)
-				ac[x] = RA ? mma(f[e], q[e], ac[x]) :
-				    mma(q[e], f[e], ac[x]);
-		}
+		for (int x = 0; x < TCW; ++x)
+			ac[x] = RA ? mma_b<F>(f, pair_of(q, x), ac[x]) :
+			    mma_b<F>(pair_of(q, x), f, ac[x]);
 	}
+	sched_fence();
 }
 
 /*
- * Phase 2 (block x = 16 cb + b, D[a][c]): role c: A = Q[a][b], B = S[c];
- * always type-B rows.
+ * Phase 2 (block x = 16 cb + b, D[a][c]): role c: A = Q[a][b], B = S[c]
+ * (rows wc + 16 cb + ln); always type-B rows; reads, mid(), MFMAs as in
+ * phase 1.
  */
-template <int F, int TCW>
+template <int F, int TCW, class Mid>
 __device__ __forceinline__ void
 mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
-    int wa, int wb, int ln, int lg)
+    int wa, int wb, int wc, int ln, int lg, Mid mid)
 {
-	typedef typename Form<F>::L L;
 	constexpr int NCB = TCW / 16;
-	const char *pv = sQ + (16 * wa + ln) * Stg::Q_USTR_B + 16 * wb * 32 +
+	static_assert(NCB == 1 || NCB == 2, "16 or 32 c per wave");
+	const char *pv = sQ + (16 * wa + ln) * Stg<4, 2>::Q_USTR_B + 16 * wb * 32 +
 	    8 * lg;
-	if constexpr (Form<F>::SPLIT) {
-		v4u f[NCB];
+	const char *pf = sS + (wc + ln) * Stg<4, 2>::S_ROW_B + 8 * lg;
+	v2u fr[2];
+	v4u q[8];
+	if constexpr (NCB == 1)
+		rd_b16(pf, pv, fr[0], q);
+	else
+		rd_b16s2(pf, pv, fr[0], fr[1], q);
+	mid();
+	sched_fence();
 #pragma unroll
-		for (int cb = 0; cb < NCB; ++cb)
-			f[cb] = split_rd<false>(sS + (16 * cb + ln) * Stg::S_ROW_B +
-			    8 * lg);
+	for (int cb = 0; cb < NCB; ++cb) {
+		const v4u fw = s_operand<F, false>(v4u{fr[cb][0], fr[cb][1], 0u,
+		    0u});
+		const v2u f = v2u{fw[0], fw[1]};
 #pragma unroll
-		for (int b = 0; b < 16; ++b) {
-			const v4u q = split_rd<false>(pv + 32 * b);
-#pragma unroll
-			for (int cb = 0; cb < NCB; ++cb)
-				ac[16 * cb + b] = mma_split<F>(q, f[cb], ac[16 * cb + b]);
-		}
-	} else {
-		typedef typename Chunk<L, 8>::V V;
-		constexpr int NE = Chunk<L, 8>::N;
-		V f[NCB];
-#pragma unroll
-		for (int cb = 0; cb < NCB; ++cb)
-			f[cb] = *(const V *)(sS + (16 * cb + ln) * Stg::S_ROW_B +
-			    8 * lg);
-#pragma unroll
-		for (int b = 0; b < 16; ++b) {
-			const V q = *(const V *)(pv + 32 * b);
-#pragma unroll
-			for (int cb = 0; cb < NCB; ++cb)
-#pragma unroll
-				for (int e = 0; e < NE; ++e)
-					ac[16 * cb + b] = mma(q[e], f[cb][e],
-					    ac[16 * cb + b]);
-		}
+		for (int b = 0; b < 16; ++b)
+			ac[16 * cb + b] = mma_b<F>(pair_of(q, b), f, ac[16 * cb + b]);
 	}
+	sched_fence();
 }
 
 /* -------------------------------------------------------------- kernel */
 
-/* c tile width of a form and W-set count (as pt_tcw) */
-constexpr int
-tcw_of(int F, int NSET)
-{
-	return NSET > 0 && F == PT_FORM_FP32 ? 32 : 16;
-}
-
-/* Every tile kernel targets two workgroups per CU (PT_WG_PER_CU): one
- * workgroup's LDS (two slots, 76 KB) and registers (<= 256 per lane) allow
- * it, and one workgroup's staging and barriers then overlap the other's
- * MFMAs.  Two W sets keep one set of accumulators: W of set 0 waits in
- * the workgroup's save area while set 1 is formed. */
+/*
+ * Workgroup geometry, per wave count NW (pt_tc, pt_cw in pt_engine.h give
+ * the same numbers to the host):
+ *   NW = 4: tile 32 a x 32 b x TC c, TC = 32 for FP32 (whose 32 accumulator
+ *           blocks of 4 registers fit beside a step's operands), 16 for FP64
+ *           and the split forms; each wave owns 16 a x 16 b x TC c; two
+ *           slots, two workgroups per CU, so one's staging and barriers
+ *           overlap the other's MFMAs.
+ *   NW = 8: tile 32 a x 32 b x 32 c, each wave 16 a x 16 b x 16 c (wave =
+ *           wa + 2 wb + 4 wc); one workgroup per CU with four slots (three
+ *           steps in flight).  Every step is type B (32 w per u-row); the
+ *           role-c operand is reused by 32 c instead of 16, and the CU holds
+ *           the same accumulators as two 4-wave workgroups.
+ */
+template <int F, int NSET, int NW>
+struct Geo {
+	static constexpr int TC = NW == 8 ? 32 : (F == PT_FORM_FP32 ? 32 : 16);
+	static constexpr int CW = NW == 8 ? 16 : TC;   /* c per wave */
+	static constexpr int THREADS = 64 * NW;
+	static constexpr int NS = NW == 8 ? 4 : 2;     /* staging slots */
+	static constexpr int WG_PER_CU = NW == 8 ? 1 : 2;
+	static constexpr int QR = 32 / NW;             /* Q runs per wave */
+};
 
 /*
  * One work item (ijk, tile) of a spin case: everything between reading the
  * item and writing its energy to partial[it].  KIND is the spin case,
  * PT_AAA or PT_AAB (only the epilogue depends on it).  With two W sets the
  * sets are formed one after the other; wsv is the item's save area for W
- * of set 0 (4 TCW values per lane per wave).
+ * of set 0 (4 CW values per lane per wave).
  */
-template <int F, int KIND, int NSET>
+template <int F, int KIND, int NSET, int NW>
 __device__ __forceinline__ void
 tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 {
 	typedef Form<F> FM;
+	typedef Geo<F, NSET, NW> GM;
+	typedef Stg<NW, GM::NS> SG;
 	typedef typename FM::G G;
-	typedef typename FM::L L;
 	typedef typename FM::AE AE;
 	typedef typename FM::ACC ACC;
 	typedef typename FM::ER ER;
 	constexpr int KC = FM::KC;
-	constexpr int TCW = tcw_of(F, NSET);
-	constexpr int NCB = TCW / 16;
+	constexpr int TC = GM::TC, CW = GM::CW;
+	constexpr int NCB = CW / 16;
+	constexpr int NT = GM::THREADS;
 	constexpr int GB = 32;  /* bytes per packed kappa group and pair */
-	/* S elements per thread per step */
-	constexpr int SEPT = F == PT_FORM_FP64 ? 1 : 2;
-	constexpr int SMEM = Stg::BYTES;
-	static_assert(SMEM >= 4 * NSET * TCW * 64 * (int)sizeof(AE),
+	constexpr int SMEM = SG::BYTES;
+	static_assert(SMEM >= NW * NSET * CW * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
+	static_assert(SG::SLOT >= 4 * 4 * 16 * 17 * (int)sizeof(AE),
+	    "transpose round exceeds a slot");
 	static_assert(NSET == 1 || NSET == 2, "one or two W sets");
 
 	__shared__ __attribute__((aligned(16))) char smem[SMEM];
-	__shared__ double red[4];
+	__shared__ double red[NW];
 	__shared__ pt_term sterm[PT_MAX_TERMS];
 	AE *sX = (AE *)smem;
 
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int wa = wave & 1, wb = wave >> 1;
+	const int wa = wave & 1, wb = (wave >> 1) & 1, wcx = wave >> 2;
+	const int wc = 16 * wcx;       /* the wave's first c in the tile */
 	const int ln = lane & 15, lg = lane >> 4;
 
 	const bool same = KIND == PT_AAA;
@@ -638,22 +786,22 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	const int i = p.ijk[3 * ijk_id], j = p.ijk[3 * ijk_id + 1];
 	const int k = p.ijk[3 * ijk_id + 2];
 	const int A0 = p.tiles[3 * tile] * 32, B0 = p.tiles[3 * tile + 1] * 32;
-	const int C0 = p.tiles[3 * tile + 2] * TCW;
+	const int C0 = p.tiles[3 * tile + 2] * TC;
 	const int na = p.na, nc = p.nc;
-	const int a_lo = A0 + 16 * wa, b_lo = B0 + 16 * wb;
+	const int a_lo = A0 + 16 * wa, b_lo = B0 + 16 * wb, c_lo = C0 + wc;
 	/* does this wave's block hold any element of the triangle? */
 	const bool act = a_lo < na && b_lo < na && a_lo + 15 > b_lo &&
-	    (!same || b_lo + 15 > C0);
+	    c_lo < nc && (!same || b_lo + 15 > c_lo);
 
-	ACC acc[TCW];
+	ACC acc[CW];
 #pragma unroll
-	for (int x = 0; x < TCW; ++x)
+	for (int x = 0; x < CW; ++x)
 		acc[x] = ACC{0, 0, 0, 0};
 	/* save area of W set 0: value (r, x) of (wave, lane) at element
-	 * ((r TCW + x) 4 + wave) 64 + lane */
-	constexpr int WST = 4 * 64 * (int)sizeof(AE);
+	 * ((r CW + x) NW + wave) 64 + lane */
+	constexpr int WST = NW * 64 * (int)sizeof(AE);
 	const __amdgpu_buffer_rsrc_t wr = brsrc(wsv, NSET == 2 ?
-	    (int64_t)4 * TCW * WST : 0);
+	    (int64_t)4 * CW * WST : 0);
 	const uint32_t wvo = (uint32_t)((wave * 64 + lane) * sizeof(AE));
 
 	/* the term table, copied once into LDS: a term change then reads it
@@ -665,30 +813,40 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		const uint32_t *src = (const uint32_t *)p.terms;
 		uint32_t *dst = (uint32_t *)sterm;
 		const int nw = nterm * (int)(sizeof(pt_term) / 4);
-		for (int w = tid; w < nw; w += PT_THREADS)
+		for (int w = tid; w < nw; w += NT)
 			dst[w] = src[w];
 		__syncthreads();
 	}
 	constexpr int ES = (int)sizeof(G);
 	auto step_of = [&](int tn) {
-		return make_step<ES, GB>(p, sterm[tn], i, j, k, A0, B0, C0, TCW);
+		return make_step<ES, GB>(p, sterm[tn], i, j, k, A0, B0, C0, TC);
 	};
 
-	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r */
+	/* [a][b]@c -> [a][c]@b through LDS scratch, per register r; with eight
+	 * waves in two rounds of four (one slot holds four waves' scratch).
+	 * Every wave passes every barrier. */
 	auto transpose = [&](AE *scratch) {
+		constexpr int NR = NW / 4;
 #pragma unroll
 		for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-		for (int r = 0; r < 4; ++r) {
-			AE *X = scratch + wave * (4 * 16 * 17);
-			__syncthreads();
+		for (int r = 0; r < 4; ++r)
 #pragma unroll
-			for (int c = 0; c < 16; ++c)
-				X[(lg * 16 + ln) * 17 + c] = acc[16 * cb + c][r];
+		for (int g = 0; g < NR; ++g) {
+			const bool mine = (wave >> 2) == g;
+			AE *X = scratch + (wave & 3) * (4 * 16 * 17);
 			__syncthreads();
+			if (mine) {
 #pragma unroll
-			for (int b = 0; b < 16; ++b)
-				acc[16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
+				for (int c = 0; c < 16; ++c)
+					X[(lg * 16 + ln) * 17 + c] = acc[16 * cb + c][r];
+			}
+			__syncthreads();
+			if (mine) {
+#pragma unroll
+				for (int b = 0; b < 16; ++b)
+					acc[16 * cb + b][r] = X[(lg * 16 + b) * 17 + ln];
+			}
 		}
 		__syncthreads();
 	};
@@ -696,41 +854,34 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	{
 		/*
 		 * Steps n = 0 .. N-1 walk the terms (role a, role b, role c) and
-		 * their kappa chunks; step n lives in slot n % NSLOT.  The loads of
-		 * step n + D (D = NSLOT - 1: Q by DMA, S into registers) are issued
-		 * while step n computes; S values wait in a register ring and are
-		 * written to their slot one step before use.  One barrier per step.
+		 * their kappa chunks; step n lives in slot n % NSLOT.  Iteration n
+		 * issues the DMAs (Q and S) of step n + D (D = NSLOT - 1) into the
+		 * slot step n - 1 used, computes step n, waits until this wave's
+		 * DMAs of step n + 1 have landed and meets the other waves at a
+		 * barrier without a memory fence, so that the DMAs of steps n + 2
+		 * .. n + D stay in flight across it.
 		 */
-		constexpr int NS = Stg::NSLOT, D = NS - 1;
+		constexpr int NS = SG::NSLOT, D = NS - 1;
 		constexpr int KSA = KC, KSB = KC / 2;   /* kappa per step, type A / B */
-		constexpr int VMS = PT_QRUN_W + SEPT;   /* loads per step per wave */
+		constexpr int VMS = GM::QR + SDma<NW>::SD; /* DMAs per step per wave */
 		const int wv = __builtin_amdgcn_readfirstlane(wave);
 		const int nta = uni(p.nterm_a), nt1 = uni(p.nterm1);
 		/* type A: a 16-wide w range (roles a and b of a 16-wide c tile) */
-		auto type_a = [&](int role) { return TCW == 16 && role != PT_ROLE_C; };
+		auto type_a = [&](int role) { return TC == 16 && role != PT_ROLE_C; };
 
 		/* prefetch walker: term pt, chunk pk of step pn (the next to issue) */
 		int pt = 0, pk = 0, pn = 0;
 		Step pre = step_of(0);
 		bool pa = type_a(pre.role);
-		uint32_t vo[PT_QRUN_W], sofs;
-		int slofs;
+		uint32_t vo[GM::QR], so[SDma<NW>::SD];
 		auto term_tables = [&]() {
 			if (pa)
-				qtab<true>(pre, wv, lane, vo);
+				qtab<true, NW>(pre, wv, lane, vo);
 			else
-				qtab<false>(pre, wv, lane, vo);
-			s_lane<F, SEPT>(pre, tid, pa, sofs, slofs);
+				qtab<false, NW>(pre, wv, lane, vo);
+			stab<F, NW>(pre, pa, wv, lane, so);
 		};
 		term_tables();
-		/* register ring: R[m] = S of step n + 1 + m (m < D - 1), its LDS
-		 * offset LR[m]; the loads of step n + D land in RN / LN */
-		constexpr int DR = D > 1 ? D - 1 : 1;
-		G R[DR][SEPT], RN[SEPT];
-		int LR[DR], LN = -1;
-#pragma unroll
-		for (int m = 0; m < DR; ++m)
-			LR[m] = -1;
 		/* number of steps */
 		int N = 0;
 		for (int tt = 0; tt < nterm; ++tt) {
@@ -738,19 +889,16 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 			const int ks = type_a(uni(sterm[tt].role)) ? KSA : KSB;
 			N += (kk + ks - 1) / ks;
 		}
-		/* issue step pn into its slot; its S goes to RN */
-		auto issue = [&]() {
-			char *slot = smem + (pn % NS) * Stg::SLOT;
-			const int ks = pa ? KSA : KSB;
+		/* issue step pn into its slot */
+		auto issue = [&](char *slot) {
 			if (pa)
-				issue_q<true>(pre, pk / KSA, slot + Stg::S_BYTES, wv, vo);
+				issue_q<true, NW>(pre, pk / KSA, slot + SG::S_BYTES, wv, vo);
 			else
-				issue_q<false>(pre, pk / KSB, slot + Stg::S_BYTES, wv, vo);
-			const int k0 = (tid % (pa ? 8 : 4)) * SEPT;
-			load_s<F, SEPT>(pre, sofs, k0, pk, ks, RN);
-			LN = slofs;
+				issue_q<false, NW>(pre, pk / KSB, slot + SG::S_BYTES, wv,
+				    vo);
+			issue_s<F, NW>(pre, pk, slot, wv, so);
 			++pn;
-			pk += ks;
+			pk += pa ? KSA : KSB;
 			if (pk >= pre.K) {
 				pk = 0;
 				if (++pt < nterm) {
@@ -760,24 +908,8 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 				}
 			}
 		};
-		/* after step n: the ring moves one step on, RN joins at the end */
-		auto rotate = [&]() {
-			if constexpr (D > 1) {
-#pragma unroll
-				for (int m = 0; m + 1 < DR; ++m) {
-#pragma unroll
-					for (int e = 0; e < SEPT; ++e)
-						R[m][e] = R[m + 1][e];
-					LR[m] = LR[m + 1];
-				}
-#pragma unroll
-				for (int e = 0; e < SEPT; ++e)
-					R[DR - 1][e] = RN[e];
-				LR[DR - 1] = LN;
-			}
-		};
-		/* wait until the loads of step m (issued) are complete: at most
-		 * (pn - 1 - m) later steps' loads may stay outstanding */
+		/* wait until this wave's DMAs of step m (issued) have landed: at
+		 * most (pn - 1 - m) later steps' DMAs may stay outstanding */
 		auto wait_step = [&](int m) {
 			const int later = pn - 1 - m;
 			if (D >= 3 && later >= 2)
@@ -787,61 +919,67 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 			else
 				wait_vm<0>();
 		};
+		/* the other waves' DMAs of that step: a barrier that waits on no
+		 * memory operation (the compiler's barrier with a fence would wait
+		 * for every DMA in flight) */
+		auto step_barrier = [&]() {
+			__asm__ volatile("" ::: "memory");
+			__builtin_amdgcn_s_barrier();
+			__asm__ volatile("" ::: "memory");
+		};
 
-		/* prologue: steps 0 .. D-1 in flight, step 0 staged; entering
-		 * iteration n the ring holds R[m] = S of step n + 1 + m */
-		{
-			G r0[SEPT];
-			issue();
-#pragma unroll
-			for (int e = 0; e < SEPT; ++e)
-				r0[e] = RN[e];
-			const int l0 = LN;
-#pragma unroll
-			for (int m = 0; m + 1 < D; ++m) {
-				if (pn < N) {
-					issue();
-#pragma unroll
-					for (int e = 0; e < SEPT; ++e)
-						R[m][e] = RN[e];
-					LR[m] = LN;
-				}
-			}
-			wait_step(0);
-			store_s<F, SEPT>(smem, l0, r0);
-			__syncthreads();
-		}
+		/* prologue: steps 0 .. D-1 in flight, step 0 landed */
+		for (int m = 0; m < D; ++m)
+			if (pn < N)
+				issue(smem + (pn % NS) * SG::SLOT);
+		wait_step(0);
+		step_barrier();
 
-		/* compute walker: term ct, chunk ck of step n */
+		/* compute walker: term ct, chunk ck of step n; the accumulators
+		 * hold W negated while cneg is set (the sign of the terms being
+		 * added) */
 		int ct = 0, ck = 0, cK = uni(sterm[0].K);
+		bool cneg = false;
 		int n = 0;
+		auto set_sign = [&](bool neg) {
+			if (neg != cneg) {
+#pragma unroll
+				for (int x = 0; x < CW; ++x)
+					acc[x] = -acc[x];
+				cneg = neg;
+			}
+		};
 		/* steps of phase PH up to term tend (exclusive) */
 		auto run_phase = [&](auto ph, int tend) {
 			constexpr int PH = decltype(ph)::value;
-			constexpr bool TA = PH < 2 && TCW == 16;
+			constexpr bool TA = PH < 2 && TC == 16;
 			while (ct < tend) {
-				/* loads of step n + D into the slot step n - 1 used */
-				if (pn < N) {
-					issue();
-				}
-				const char *slot = smem + (n % NS) * Stg::SLOT;
-				const char *cS = slot, *cQ = slot + Stg::S_BYTES;
-				if constexpr (PH < 2)
-					mma_p1<F, TCW, PH == 0, TA>(acc, cS, cQ, wa, wb, ln,
-					    lg);
-				else
-					mma_p2<F, TCW>(acc, cS, cQ, wa, wb, ln, lg);
-				/* stage the S rows of step n + 1 */
-				if (n + 1 < N) {
-					wait_step(n + 1);
-					char *sn = smem + ((n + 1) % NS) * Stg::SLOT;
-					if constexpr (D > 1)
-						store_s<F, SEPT>(sn, LR[0], R[0]);
+				set_sign(uni(sterm[ct].sign) < 0);
+				/* step n reads slot rsl; after its operand reads, the
+				 * DMAs of step n + D go to dsl, the slot step n - 1 used.
+				 * The two never overlap within a step, which the
+				 * restrict qualifiers tell the compiler, so that it
+				 * places no wait for those DMAs before the reads; the
+				 * order across steps comes from wait_step and the
+				 * barrier. */
+				auto body = [&](const char *__restrict__ rsl,
+				    char *__restrict__ dsl) {
+					auto mid = [&]() {
+						if (pn < N)
+							issue(dsl);
+					};
+					if constexpr (PH < 2)
+						mma_p1<F, CW, PH == 0, TA>(acc, rsl,
+						    rsl + SG::S_BYTES, wa, wb, wc, ln, lg, mid);
 					else
-						store_s<F, SEPT>(sn, LN, RN);
-					rotate();
-				}
-				__syncthreads();
+						mma_p2<F, CW>(acc, rsl, rsl + SG::S_BYTES, wa, wb,
+						    wc, ln, lg, mid);
+				};
+				body(smem + (n % NS) * SG::SLOT,
+				    smem + (pn % NS) * SG::SLOT);
+				if (n + 1 < N)
+					wait_step(n + 1);
+				step_barrier();
 				++n;
 				ck += TA ? KSA : KSB;
 				if (ck >= cK) {
@@ -850,6 +988,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 						cK = uni(sterm[ct].K);
 				}
 			}
+			set_sign(false);
 		};
 #pragma unroll 1
 		for (int s = 0; s < NSET; ++s) {
@@ -858,7 +997,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 			run_phase(std::integral_constant<int, 1>(), t0 + nt1);
 			/* the slot of the last phase-1 step is free until step
 			 * n + D issues into it: transpose through it */
-			transpose((AE *)(smem + ((n + NS - 1) % NS) * Stg::SLOT));
+			transpose((AE *)(smem + ((n + NS - 1) % NS) * SG::SLOT));
 			run_phase(std::integral_constant<int, 2>(), t0 + nset_terms);
 			if (NSET == 2 && s == 0) {
 				/* W of set 0 to the save area, lane-contiguous per
@@ -866,29 +1005,32 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 #pragma unroll
 				for (int r = 0; r < 4; ++r)
 #pragma unroll
-					for (int x = 0; x < TCW; ++x) {
-						bstore_ae<AE>(wr, wvo, (r * TCW + x) * WST,
+					for (int x = 0; x < CW; ++x) {
+						bstore_ae<AE>(wr, wvo, (r * CW + x) * WST,
 						    acc[x][r]);
 						acc[x][r] = 0;
 					}
+				/* the waits of later steps count outstanding
+				 * memory operations; let none of them be a store */
+				wait_vm<0>();
 			}
 		}
 	}
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
-	 * c = C0 + 16 cb + ln.  Each register column r goes through this
+	 * c = C0 + wc + 16 cb + ln.  Each register column r goes through this
 	 * lane's own LDS slots so the element loop runs with a runtime index. */
 	double esum = 0.0, ecmp = 0.0;
-	AE *E = sX + wave * (NSET * TCW * 64) + lane;
+	AE *E = sX + wave * (NSET * CW * 64) + lane;
 #pragma unroll
 	for (int r = 0; r < 4; ++r) {
 		__syncthreads();
 #pragma unroll
-		for (int x = 0; x < TCW; ++x) {
+		for (int x = 0; x < CW; ++x) {
 			E[x * 64] = acc[x][r];
 			if (NSET == 2)
-				E[(TCW + x) * 64] = bload_ae<AE>(wr, wvo,
-				    (r * TCW + x) * WST);
+				E[(CW + x) * 64] = bload_ae<AE>(wr, wvo,
+				    (r * CW + x) * WST);
 		}
 		if (!act)
 			continue;
@@ -896,9 +1038,9 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		const int a = a_lo + arow;
 		if (a >= na)
 			continue;
-		for (int x = 0; x < TCW; ++x) {
+		for (int x = 0; x < CW; ++x) {
 			const int b = b_lo + (x & 15);
-			const int c = C0 + 16 * (x >> 4) + ln;
+			const int c = c_lo + 16 * (x >> 4) + ln;
 			if (!(c < nc && a > b && (!same || b > c)))
 				continue;
 			ER z, dn;
@@ -923,7 +1065,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 					w *= (ER)p.inv_scale[0];
 				term = w * (w - z) / dn;
 			} else {
-				ER wl = (ER)E[(TCW + x) * 64], wr = (ER)E[x * 64];
+				ER wl = (ER)E[(CW + x) * 64], wr = (ER)E[x * 64];
 				if (FM::SPLIT) {
 					wl *= (ER)p.inv_scale[0];
 					wr *= (ER)p.inv_scale[1];
@@ -942,16 +1084,20 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	if (lane == 0)
 		red[wave] = e;
 	__syncthreads();
-	if (tid == 0)
-		p.partial[it] = (red[0] + red[1]) + (red[2] + red[3]);
+	if (tid == 0) {
+		double t = (red[0] + red[1]) + (red[2] + red[3]);
+		if constexpr (NW == 8)
+			t += (red[4] + red[5]) + (red[6] + red[7]);
+		p.partial[it] = t;
+	}
 }
 
 /* save area `slot` of wsave (two W sets; pt_wsave_bytes each) */
-template <int F>
+template <int F, int NW>
 __device__ __forceinline__ typename Form<F>::AE *
 wsave_of(void *wsave, int64_t slot)
 {
-	constexpr int64_t bytes = (int64_t)4 * tcw_of(F, 2) * 4 * 64 *
+	constexpr int64_t bytes = (int64_t)NW * Geo<F, 2, NW>::CW * 4 * 64 *
 	    sizeof(typename Form<F>::AE);
 	return (typename Form<F>::AE *)((char *)wsave + slot * bytes);
 }
@@ -964,16 +1110,16 @@ wsave_of(void *wsave, int64_t slot)
  * W sets item item0 + n keeps W of set 0 in save area n (n_chunk <=
  * p.nwsave).
  */
-template <int F, int KIND, int NSET>
-__global__ __launch_bounds__(PT_THREADS, PT_WG_PER_CU) void
+template <int F, int KIND, int NSET, int NW>
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 1 : 2) void
 k_tiles(pt_launch p)
 {
 	const int64_t per = (p.n_chunk + 7) / 8;
 	const int64_t x = blockIdx.x & 7;
 	const int64_t it = x * per + (blockIdx.x >> 3);
 	if (it < p.n_chunk)
-		tile_item<F, KIND, NSET>(p, p.item0 + it,
-		    NSET == 2 ? wsave_of<F>(p.wsave, it) : nullptr);
+		tile_item<F, KIND, NSET, NW>(p, p.item0 + it,
+		    NSET == 2 ? wsave_of<F, NW>(p.wsave, it) : nullptr);
 }
 
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
@@ -1143,73 +1289,47 @@ k_synth(pt_synth_spec spec, int tensor, void *dst, int64_t n, int fp32)
 	}
 }
 
-template <int F, int NSET>
+template <int F, int NSET, int NW>
 int
 launch_kind(const pt_launch *p, hipStream_t s)
 {
-	/* one save area per item */
-	if (NSET == 2 && (!p->wsave || p->n_chunk > p->nwsave))
+	typedef Geo<F, NSET, NW> GM;
+	/* one save area per item; the host's tile width must be the kernel's */
+	if ((NSET == 2 && (!p->wsave || p->n_chunk > p->nwsave)) ||
+	    p->tcw != GM::TC)
 		return (int)hipErrorInvalidValue;
-	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(PT_THREADS);
+	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(GM::THREADS);
 	if (p->kind == PT_AAA)
-		k_tiles<F, PT_AAA, NSET><<<grid, block, 0, s>>>(*p);
+		k_tiles<F, PT_AAA, NSET, NW><<<grid, block, 0, s>>>(*p);
 	else
-		k_tiles<F, PT_AAB, NSET><<<grid, block, 0, s>>>(*p);
+		k_tiles<F, PT_AAB, NSET, NW><<<grid, block, 0, s>>>(*p);
 	return (int)hipGetLastError();
-}
-
-/* Kernels built in pt_kernel_agpr.hip, the same source compiled without
- * the VGPR-form MFMA option: FP64 with two W sets (the option gave wrong
- * energies when that kernel held both sets in registers; with the sets
- * formed one after the other it is correct there but no faster) and the
- * split forms (whose accumulators stay in place either way, and for which
- * the option can crash the compiler). */
-constexpr bool
-agpr_unit(int F, int NSET)
-{
-	return (F == PT_FORM_FP64 && NSET == 2) || F >= PT_FORM_BF16X3;
 }
 
 template <int F>
 int
 launch_form(const pt_launch *p, hipStream_t s)
 {
-	if (p->nset == 1) {
-		if constexpr (!agpr_unit(F, 1) || PT_KERNEL_AGPR)
-			return launch_kind<F, 1>(p, s);
-	} else {
-		if constexpr (!agpr_unit(F, 2) || PT_KERNEL_AGPR)
-			return launch_kind<F, 2>(p, s);
-	}
-	return (int)hipErrorInvalidValue;
+	if (p->nw == 8)
+		return p->nset == 1 ? launch_kind<F, 1, 8>(p, s) :
+		    launch_kind<F, 2, 8>(p, s);
+	return p->nset == 1 ? launch_kind<F, 1, 4>(p, s) :
+	    launch_kind<F, 2, 4>(p, s);
 }
 
 } /* namespace */
 
-#if PT_KERNEL_AGPR
-extern "C" int
-pt_launch_tiles_agpr(int form, const struct pt_launch *p, void *stream)
-{
-	hipStream_t s = (hipStream_t)stream;
-	switch (form) {
-	case PT_FORM_FP64: return launch_kind<PT_FORM_FP64, 2>(p, s);
-	case PT_FORM_BF16X3: return launch_form<PT_FORM_BF16X3>(p, s);
-	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
-	}
-	return (int)hipErrorInvalidValue;
-}
-#else
 extern "C" int
 pt_launch_tiles(int form, const struct pt_launch *p, void *stream)
 {
 	hipStream_t s = (hipStream_t)stream;
 	if (p->n_chunk <= 0)
 		return 0;
-	if (agpr_unit(form, p->nset))
-		return pt_launch_tiles_agpr(form, p, stream);
 	switch (form) {
 	case PT_FORM_FP64: return launch_form<PT_FORM_FP64>(p, s);
 	case PT_FORM_FP32: return launch_form<PT_FORM_FP32>(p, s);
+	case PT_FORM_BF16X3: return launch_form<PT_FORM_BF16X3>(p, s);
+	case PT_FORM_F16X3: return launch_form<PT_FORM_F16X3>(p, s);
 	}
 	return (int)hipErrorInvalidValue;
 }
@@ -1299,4 +1419,3 @@ pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n, int fp32,
 	    fp32);
 	return (int)hipGetLastError();
 }
-#endif /* PT_KERNEL_AGPR */

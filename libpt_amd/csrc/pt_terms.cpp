@@ -12,8 +12,6 @@
  */
 #include <string.h>
 
-#include <vector>
-
 #include "pt_engine.h"
 #include "pt_terms.h"
 
@@ -74,7 +72,7 @@ struct TermBuilder {
 } /* namespace */
 
 void
-pt_terms_same(std::vector<pt_term> &out, int set, int64_t o, int64_t v)
+pt_terms_same(pt_term_list &out, int set, int64_t o, int64_t v)
 {
 	const int bT = 12 * set + 0, bO = 12 * set + 3, bV = 12 * set + 6;
 	const int bTq = 12 * set + 9;
@@ -119,7 +117,7 @@ pt_terms_same(std::vector<pt_term> &out, int set, int64_t o, int64_t v)
 }
 
 void
-pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
+pt_terms_mixed(pt_term_list &out, int set, int64_t oa, int64_t va,
     int64_t ob, int64_t vb)
 {
 	const int b0 = 12 * set;
@@ -189,7 +187,7 @@ pt_terms_mixed(std::vector<pt_term> &out, int set, int64_t oa, int64_t va,
 }
 
 int
-pt_terms_order(std::vector<pt_term> &terms, int *n_role_a)
+pt_terms_order(pt_term_list &terms, int *n_role_a)
 {
 	/* W set 0, then W set 1; within a set role a, role b, role c, each
 	 * group in table order.  Both sets have the same structure, so the
@@ -197,7 +195,7 @@ pt_terms_order(std::vector<pt_term> &terms, int *n_role_a)
 	int nset = 1;
 	for (const pt_term &t : terms)
 		nset = t.set + 1 > nset ? t.set + 1 : nset;
-	std::vector<pt_term> out;
+	pt_term_list out;
 	int na = 0, n1 = 0;
 	for (int s = 0; s < nset; ++s)
 		for (int role = PT_ROLE_A; role <= PT_ROLE_C; ++role)

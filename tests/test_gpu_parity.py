@@ -87,6 +87,21 @@ def test_split_forms(form, tol, kind, dims, kw):
     assert rel <= tol
 
 
+@pytest.mark.parametrize("form", ["bf16x3", "f16x3"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_split_forms_on_fixtures(golden, name, form):
+    """The split forms on libpt's own molecular inputs (physical
+    amplitudes and integrals, |E| of order 1e-3 Eh): within 1e-6 Eh of the
+    known answer and of the FP64 oracle."""
+    fx = golden[name]
+    e = api.energy(fx.kind, form, dims_of(fx), fx.args())
+    eo = oracle.energy(fx.kind, dims_of(fx), fx.args())
+    print(name, form, "gpu", e, "oracle", eo, "golden", fx.e_ref,
+          "|dE| vs oracle", abs(e - eo))
+    assert abs(e - eo) <= 1e-6
+    assert abs(e - fx.e_ref) <= 1e-6
+
+
 def test_restricted_equals_embedded_unrestricted():
     o, v = 5, 21
     arr = _synth("rpt", (o, v), sig_f=1e-2)
@@ -189,3 +204,22 @@ def test_fp32_against_compiled_reference(kind, dims, kw):
     print(kind, dims, "gpu _mp", e, "reference _mp", er, "rel",
           abs(e - er) / abs(er))
     assert abs(e - er) <= 1e-5 * abs(er)
+
+
+def test_dropin_stages_large_inputs_through_pinned_buffers():
+    """Inputs above 256 MB go to HBM through the pinned, multi-threaded
+    staging path of the drop-in: the energy equals that of a plan on the
+    same arrays copied with hipMemcpy, bit for bit."""
+    kind, dims = "rpt", (10, 180)
+    arr = _synth(kind, dims, seed=7)
+    assert sum(a.nbytes for a in arr) > 256 << 20
+    e = api.rpt(*dims, *arr)
+    dev = api.to_device(kind, arr)
+    try:
+        with api.Plan(kind, "fp64", dims, dev) as pl:
+            pl.run_all()
+            ep = pl.energy()
+    finally:
+        dev.close()
+    print("drop-in", e, "plan", ep)
+    assert e == ep

@@ -71,3 +71,18 @@ def test_sizes_match_api(kind, dims):
     spec = api.make_spec(kind, dims)
     arr = api.synth_host(kind, spec)
     assert [len(a) for a in arr] == api_sizes(kind, dims)
+
+
+def test_range_is_a_slice_of_the_whole_tensor():
+    spec = api.make_spec("rft", (5, 9), seed=3, sig_f=1e-2, omega=0.3)
+    for t, whole in zip(api.tensor_ids("rft"), api.synth_host("rft", spec)):
+        for b, n in ((0, whole.size), (7, 13), (whole.size - 5, 5),
+                     (whole.size, 0)):
+            part = np.empty(n)
+            api.synth_range(spec, t, part, b)
+            assert np.array_equal(part, whole[b:b + n])
+        part = np.empty(4, dtype=np.float32)
+        api.synth_range(spec, t, part, 2)
+        assert np.array_equal(part, whole[2:6].astype(np.float32))
+        with pytest.raises(api.LibptError):
+            api.synth_range(spec, t, np.empty(2), whole.size - 1)
