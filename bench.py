@@ -130,7 +130,10 @@ def run_form(api, form, dims, dev, args, nbatch):
     dom = max(range(ncase), key=lambda sc: nijk[sc] * fpi[sc])
     dom_flop = sum((rng(dom, b)[1] - rng(dom, b)[0]) * fpi[dom]
                    for b in batches)
+    waves, tcw, kgrp = plan.geometry()
     res = dict(form=form, prep_s=prep, tile_block=blk, tune_s=tune_s,
+               geometry={"waves": waves, "tile": f"32x32x{tcw}",
+                         "ijk_group": kgrp},
                wall_s=wall, flop=flop,
                tflops=flop / wall / 1e12, ms_per_step=1e3 * wall / args.steps,
                kern_ms=kern_ms, dom_case=dom,
@@ -609,6 +612,7 @@ def main():
             "total_flop": r["total_flop"],
             "projected_full_T_wall_s": round(proj_wall, 1),
             "tile_block": r["tile_block"],
+            "geometry": r["geometry"],
         },
         "roofline": {
             "bound": "mfma",
@@ -627,6 +631,7 @@ def main():
             "ms_per_step": round(q["ms_per_step"], 2),
             "speedup_vs_" + main_form: round(q["tflops"] / r["tflops"], 3),
             "tile_block": q["tile_block"],
+            "geometry": q["geometry"],
             "projected_full_T_wall_s": round(q["total_flop"] /
                                              (q["tflops"] * 1e12), 1),
             "mfma_frac": round(q["dom_flop_per_launch"] * EXEC_FACTOR[form] /

@@ -66,6 +66,12 @@ int64_t libpt_amd_plan_items(const libpt_amd_plan *plan, int sc);
 double libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *plan, int sc);
 /* multiplier of spin case sc in the call's energy (2 for restricted) */
 double libpt_amd_plan_factor(const libpt_amd_plan *plan, int sc);
+/* tile-kernel geometry of the plan: out3 = (waves per workgroup, c tile
+ * width, ijk per item group).  Defaults: 8 waves (32 x 32 x 32 tiles) for
+ * FP64 and the split forms, 4 waves (32 x 32 x 32 with 32 c per wave) for
+ * FP32; no grouping of consecutive ijk (1).  LIBPT_AMD_WAVES=4|8 and
+ * LIBPT_AMD_KGROUP=n override them for new plans. */
+int libpt_amd_plan_geometry(const libpt_amd_plan *plan, int *out3);
 
 /* enqueue ijk items [ijk_begin, ijk_end) of spin case sc on the plan's
  * stream; 0 on success */

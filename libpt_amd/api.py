@@ -86,6 +86,7 @@ def lib():
             fn.restype = _d
             fn.argtypes = [_p, _i]
         L.libpt_amd_plan_run.restype = _i
+        L.libpt_amd_plan_geometry.argtypes = [_p, _p]
         L.libpt_amd_plan_run.argtypes = [_p, _i, _i64, _i64]
         L.libpt_amd_plan_sync.restype = _i
         L.libpt_amd_plan_sync.argtypes = [_p]
@@ -336,6 +337,13 @@ class Plan:
 
     def factor(self, sc):
         return lib().libpt_amd_plan_factor(self.h, sc)
+
+    def geometry(self):
+        """(waves per workgroup, c tile width, ijk per item group)"""
+        out = (C.c_int * 3)()
+        if lib().libpt_amd_plan_geometry(self.h, out) != 0:
+            raise LibptError(last_error())
+        return tuple(out)
 
     def run(self, sc, b=0, e=None):
         if e is None:

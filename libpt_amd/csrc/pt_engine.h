@@ -139,7 +139,12 @@ struct pt_launch {
 	                                of the launch (pt_wsave_bytes each,
 	                                item item0 + n at slot n), else
 	                                unused */
-	int32_t nwsave, pad2_;        /* save areas in wsave (>= n_chunk) */
+	int32_t nwsave;              /* save areas in wsave (>= n_chunk) */
+	int32_t kgrp;                /* > 1: the launch covers whole ijk
+	                                (item0 and n_chunk multiples of
+	                                n_tiles) taken in groups of kgrp
+	                                consecutive ijk, each group's items
+	                                tile-major; 1: items in order */
 	struct pt_epi epi;
 };
 

@@ -292,6 +292,7 @@ struct Arena {
 struct libpt_amd_plan {
 	int call = 0, form = 0;
 	int nw = 4;                 /* waves per workgroup of the tile kernel */
+	int kgrp = 1;               /* ijk per item group of a launch */
 	int64_t oa = 0, va = 0, ob = 0, vb = 0;
 	hipStream_t stream = nullptr;
 	bool own_stream = true;     /* created (and destroyed) by the plan */
@@ -533,6 +534,7 @@ case_finish(libpt_amd_plan *pl, Case &c, int form, int kind, int nset,
 {
 	pt_launch &L = c.L;
 	L.nw = pl->nw;
+	L.kgrp = pl->kgrp;
 	L.kind = kind;
 	L.nset = nset;
 	L.oa = oa;
@@ -646,6 +648,8 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 	pl->nw = pt_nw_default(form);
 	if (const char *w = getenv("LIBPT_AMD_WAVES"))
 		pl->nw = atoi(w) == 4 ? 4 : atoi(w) == 8 ? 8 : pl->nw;
+	if (const char *g = getenv("LIBPT_AMD_KGROUP"))
+		pl->kgrp = std::max(1, std::min(64, atoi(g)));
 	pl->call = call;
 	pl->form = form;
 	pl->oa = oa;
@@ -921,6 +925,44 @@ plan_create(int call, int form, size_t oa, size_t va, size_t ob, size_t vb,
 
 } /* namespace */
 
+namespace {
+
+/*
+ * The tile launches over items [item_b, item_e) of one spin case: grids of
+ * at most 2^22 items, and with two W sets of at most wslots (one save area
+ * per item).  With base.kgrp > 1, launches hold whole ijk (the kernel
+ * takes them in groups of kgrp consecutive ijk); where a launch cannot
+ * hold one whole ijk the items go in order.
+ */
+int
+launch_items(int form, const pt_launch &base, int64_t item_b, int64_t item_e,
+    void *wsave, int64_t wslots, hipStream_t st)
+{
+	const int64_t nt = base.n_tiles;
+	int64_t chunk = base.nset == 2 ? wslots : (int64_t)1 << 22;
+	int kg = base.kgrp;
+	if (kg > 1) {
+		if (item_b % nt || item_e % nt || chunk < nt)
+			kg = 1;
+		else
+			chunk = chunk / nt * nt;
+	}
+	for (int64_t it = item_b; it < item_e; it += chunk) {
+		pt_launch L = base;
+		L.item0 = it;
+		L.n_chunk = std::min(chunk, item_e - it);
+		L.kgrp = kg;
+		L.wsave = wsave;
+		L.nwsave = (int32_t)wslots;
+		const int rc = pt_launch_tiles(form, &L, st);
+		if (rc != 0)
+			return rc;
+	}
+	return 0;
+}
+
+} /* namespace */
+
 extern "C" libpt_amd_plan *
 libpt_amd_plan_create(int call, int form, size_t oa, size_t va, size_t ob,
     size_t vb, const void *const *arr)
@@ -955,6 +997,18 @@ libpt_amd_plan_items(const libpt_amd_plan *pl, int sc)
 	return pl->cases[sc].n_ijk * pl->cases[sc].n_tiles;
 }
 
+extern "C" int
+libpt_amd_plan_geometry(const libpt_amd_plan *pl, int *out3)
+{
+	if (!pl || !out3)
+		return -1;
+	out3[0] = pl->nw;
+	out3[1] = pt_tcw(pl->form, pl->call == LIBPT_AMD_RFT ||
+	    pl->call == LIBPT_AMD_UFT ? 2 : 1, pl->nw);
+	out3[2] = pl->kgrp;
+	return 0;
+}
+
 extern "C" double
 libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *pl, int sc)
 {
@@ -987,24 +1041,14 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 		e = c.n_ijk;
 	if (e <= b)
 		return 0;
-	const int64_t item_b = b * c.n_tiles, item_e = e * c.n_tiles;
-	const bool two = c.L.nset == 2;
-	if (two && !plan_wsave(pl))
+	if (c.L.nset == 2 && !plan_wsave(pl))
 		return -1;
-	/* grids of at most 2^22 items; two W sets: of at most wslots */
-	const int64_t chunk = two ? pl->wslots : (int64_t)1 << 22;
-	for (int64_t it = item_b; it < item_e; it += chunk) {
-		pt_launch L = c.L;
-		L.item0 = it;
-		L.n_chunk = item_e - it < chunk ? item_e - it : chunk;
-		L.wsave = pl->wsave;
-		L.nwsave = (int32_t)pl->wslots;
-		int rc = pt_launch_tiles(pl->form, &L, pl->stream);
-		if (rc != 0) {
-			set_err("tile kernel launch failed: %s",
-			    hipGetErrorString((hipError_t)rc));
-			return -1;
-		}
+	const int rc = launch_items(pl->form, c.L, b * c.n_tiles,
+	    e * c.n_tiles, pl->wsave, pl->wslots, pl->stream);
+	if (rc != 0) {
+		set_err("tile kernel launch failed: %s",
+		    hipGetErrorString((hipError_t)rc));
+		return -1;
 	}
 	return 0;
 }
@@ -1358,17 +1402,9 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) ==
 	    hipSuccess || (set_err("stream capture failed"), false));
 	int rc = 0;
-	const int64_t chunk = b->nset == 2 ? b->wslots : (int64_t)1 << 22;
-	for (int g = 0; g < nseg && rc == 0; ++g) {
-		for (int64_t it = 0; it < segs[g].items && rc == 0; it += chunk) {
-			pt_launch Lc = L[g];
-			Lc.item0 = it;
-			Lc.n_chunk = std::min<int64_t>(chunk, segs[g].items - it);
-			Lc.wsave = b->wsave;
-			Lc.nwsave = (int32_t)b->wslots;
-			rc = pt_launch_tiles(form, &Lc, b->stream);
-		}
-	}
+	for (int g = 0; g < nseg && rc == 0; ++g)
+		rc = launch_items(form, L[g], 0, segs[g].items, b->wsave, b->wslots,
+		    b->stream);
 	if (rc == 0)
 		rc = pt_launch_reduce_batch(b->d_segs, b->d_begin, nseg, b->d_out,
 		    b->stream);

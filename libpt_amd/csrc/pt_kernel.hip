@@ -40,6 +40,7 @@
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
@@ -484,7 +485,8 @@ split_words_s(v4u e)
 /*
  * Operand reads of one step from LDS, all of a wave's reads in one asm
  * block that ends by waiting for them (so the registers it returns hold
- * their data).  Reading in asm keeps the compiler from placing a wait for
+ * their data).  The outputs are early-clobber: a read's data may land
+ * while later reads of the block still need the address registers.  Reading in asm keeps the compiler from placing a wait for
  * the DMAs in flight (into other slots) before these reads: the order of
  * DMAs and reads is kept by the explicit waits and barriers of the step
  * loop.  Type-B rows (32 B per w): S is 8 bytes at pf, Q value x is 8
@@ -503,9 +505,9 @@ rd_b16(const char *pf, const char *pv, v2u &f, v4u (&q)[8])
 	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
 	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
 	    "s_waitcnt lgkmcnt(0)"
-	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
-	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
-	      [q6] "=v"(q[6]), [q7] "=v"(q[7])
+	    : [f] "=&v"(f), [q0] "=&v"(q[0]), [q1] "=&v"(q[1]), [q2] "=&v"(q[2]),
+	      [q3] "=&v"(q[3]), [q4] "=&v"(q[4]), [q5] "=&v"(q[5]),
+	      [q6] "=&v"(q[6]), [q7] "=&v"(q[7])
 	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
 }
 
@@ -518,9 +520,9 @@ rd_b16s2(const char *pf, const char *pv, v2u &f0, v2u &f1, v4u (&q)[8])
 	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
 	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
 	    "s_waitcnt lgkmcnt(0)"
-	    : [f0] "=v"(f0), [f1] "=v"(f1), [q0] "=v"(q[0]), [q1] "=v"(q[1]),
-	      [q2] "=v"(q[2]), [q3] "=v"(q[3]), [q4] "=v"(q[4]),
-	      [q5] "=v"(q[5]), [q6] "=v"(q[6]), [q7] "=v"(q[7])
+	    : [f0] "=&v"(f0), [f1] "=&v"(f1), [q0] "=&v"(q[0]), [q1] "=&v"(q[1]),
+	      [q2] "=&v"(q[2]), [q3] "=&v"(q[3]), [q4] "=&v"(q[4]),
+	      [q5] "=&v"(q[5]), [q6] "=&v"(q[6]), [q7] "=&v"(q[7])
 	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
 }
 
@@ -533,12 +535,12 @@ rd_b32(const char *pf, const char *pv, v2u &f, v4u (&q)[16])
 	    PT_R2(8, 64, 68) PT_R2(9, 72, 76) PT_R2(10, 80, 84) PT_R2(11, 88, 92)
 	    PT_R2(12, 96, 100) PT_R2(13, 104, 108) PT_R2(14, 112, 116) PT_R2(15, 120, 124)
 	    "s_waitcnt lgkmcnt(0)"
-	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
-	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
-	      [q6] "=v"(q[6]), [q7] "=v"(q[7]), [q8] "=v"(q[8]),
-	      [q9] "=v"(q[9]), [q10] "=v"(q[10]), [q11] "=v"(q[11]),
-	      [q12] "=v"(q[12]), [q13] "=v"(q[13]), [q14] "=v"(q[14]),
-	      [q15] "=v"(q[15])
+	    : [f] "=&v"(f), [q0] "=&v"(q[0]), [q1] "=&v"(q[1]), [q2] "=&v"(q[2]),
+	      [q3] "=&v"(q[3]), [q4] "=&v"(q[4]), [q5] "=&v"(q[5]),
+	      [q6] "=&v"(q[6]), [q7] "=&v"(q[7]), [q8] "=&v"(q[8]),
+	      [q9] "=&v"(q[9]), [q10] "=&v"(q[10]), [q11] "=&v"(q[11]),
+	      [q12] "=&v"(q[12]), [q13] "=&v"(q[13]), [q14] "=&v"(q[14]),
+	      [q15] "=&v"(q[15])
 	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
 }
 
@@ -552,12 +554,12 @@ rd_a16(const char *pf, const char *pv, v4u &f, v4u (&q)[16])
 	    PT_R4(7, 448) PT_R4(8, 512) PT_R4(9, 576) PT_R4(10, 640) PT_R4(11, 704) PT_R4(12, 768)
 	    PT_R4(13, 832) PT_R4(14, 896) PT_R4(15, 960)
 	    "s_waitcnt lgkmcnt(0)"
-	    : [f] "=v"(f), [q0] "=v"(q[0]), [q1] "=v"(q[1]), [q2] "=v"(q[2]),
-	      [q3] "=v"(q[3]), [q4] "=v"(q[4]), [q5] "=v"(q[5]),
-	      [q6] "=v"(q[6]), [q7] "=v"(q[7]), [q8] "=v"(q[8]),
-	      [q9] "=v"(q[9]), [q10] "=v"(q[10]), [q11] "=v"(q[11]),
-	      [q12] "=v"(q[12]), [q13] "=v"(q[13]), [q14] "=v"(q[14]),
-	      [q15] "=v"(q[15])
+	    : [f] "=&v"(f), [q0] "=&v"(q[0]), [q1] "=&v"(q[1]), [q2] "=&v"(q[2]),
+	      [q3] "=&v"(q[3]), [q4] "=&v"(q[4]), [q5] "=&v"(q[5]),
+	      [q6] "=&v"(q[6]), [q7] "=&v"(q[7]), [q8] "=&v"(q[8]),
+	      [q9] "=&v"(q[9]), [q10] "=&v"(q[10]), [q11] "=&v"(q[11]),
+	      [q12] "=&v"(q[12]), [q13] "=&v"(q[13]), [q14] "=&v"(q[14]),
+	      [q15] "=&v"(q[15])
 	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
 }
 
@@ -587,10 +589,13 @@ mma_b(v2u a, v2u b, typename Form<F>::ACC c)
 		return mma(__builtin_bit_cast(double, a),
 		    __builtin_bit_cast(double, b), c);
 	} else if constexpr (F == PT_FORM_FP32) {
-		c = mma(__builtin_bit_cast(float, a[0]),
-		    __builtin_bit_cast(float, b[0]), c);
-		return mma(__builtin_bit_cast(float, a[1]),
-		    __builtin_bit_cast(float, b[1]), c);
+		/* whole vectors are bit-cast, then indexed: a bit cast of one
+		 * element (bit_cast(float, a[1])) reads element 0 with this
+		 * compiler */
+		const v2f x = __builtin_bit_cast(v2f, a), y = __builtin_bit_cast(v2f,
+		    b);
+		c = mma(x[0], y[0], c);
+		return mma(x[1], y[1], c);
 	} else {
 		return mma_split<F, false>(v4u{a[0], a[1], 0u, 0u},
 		    v4u{b[0], b[1], 0u, 0u}, c);
@@ -1103,12 +1108,18 @@ wsave_of(void *wsave, int64_t slot)
 }
 
 /*
- * One launch over items [item0, item0 + n_chunk) of one spin case.  XCD-
- * aware order: workgroup n runs on XCD n % 8, and each XCD gets a
- * contiguous run of items, so neighbouring tiles of one ijk share L2.
- * One workgroup per item, handed out by the dispatcher in order; with two
- * W sets item item0 + n keeps W of set 0 in save area n (n_chunk <=
- * p.nwsave).
+ * One launch over items [item0, item0 + n_chunk) of one spin case (item =
+ * ijk * n_tiles + tile).  XCD-aware order: workgroup n runs on XCD n % 8,
+ * and each XCD gets a contiguous run of the launch's items, so the
+ * workgroups running together on one XCD share operand slices in L2.  With
+ * kgrp > 1 that run is taken in groups of kgrp consecutive ijk, tile-major
+ * within a group: the work list holds (i, j, k) with k fastest, and
+ * consecutive k share the slabs of i and j (six of the eight ovvv terms of
+ * the mixed-spin case), which the kgrp items of one tile then read
+ * together.  Each item still writes its own energy at its place in the
+ * work list, so the energies do not depend on kgrp.  One workgroup per
+ * item, handed out by the dispatcher in order; with two W sets item n of
+ * the launch keeps W of set 0 in save area n (n_chunk <= p.nwsave).
  */
 template <int F, int KIND, int NSET, int NW>
 __global__ __launch_bounds__(64 * NW, NW == 8 ? 1 : 2) void
@@ -1117,9 +1128,18 @@ k_tiles(pt_launch p)
 	const int64_t per = (p.n_chunk + 7) / 8;
 	const int64_t x = blockIdx.x & 7;
 	const int64_t it = x * per + (blockIdx.x >> 3);
-	if (it < p.n_chunk)
-		tile_item<F, KIND, NSET, NW>(p, p.item0 + it,
-		    NSET == 2 ? wsave_of<F, NW>(p.wsave, it) : nullptr);
+	if (it >= p.n_chunk)
+		return;
+	int64_t item = p.item0 + it;
+	if (p.kgrp > 1) {
+		const int64_t nt = p.n_tiles, G = p.kgrp, m = p.n_chunk / nt;
+		const int64_t grp = it / (G * nt), r = it - grp * G * nt;
+		const int64_t gsz = G < m - grp * G ? G : m - grp * G;
+		const int64_t tile = r / gsz, g = r - tile * gsz;
+		item = p.item0 + (grp * G + g) * nt + tile;
+	}
+	tile_item<F, KIND, NSET, NW>(p, item,
+	    NSET == 2 ? wsave_of<F, NW>(p.wsave, it) : nullptr);
 }
 
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */

@@ -1,6 +1,7 @@
 """Launch chunking of the (fT) calls.  A launch with two W sets holds at
-most 32768 items, because every item of a launch keeps W of set 0 in its
-own save area; larger spin cases run in several launches.  The energy of
+most 4 GB of save areas (16384 to 32768 items by form and tile geometry),
+because every item of a launch keeps W of set 0 in its own save area;
+larger spin cases run in several launches.  The energy of
 every ijk triple must not depend on how the work list is cut into
 launches: the same items run one ijk per launch give the same bits."""
 
@@ -16,7 +17,7 @@ FT_CHUNK = 32768
 
 @pytest.mark.parametrize("form", ["fp64", "bf16x3"])
 def test_ft_chunked_launches_match_single_ijk_runs(form):
-    kind, dims = "rft", (10, 128)
+    kind, dims = "rft", (12, 160)
     spec = api.make_spec(kind, dims, seed=3, omega=0.3, sig_f=1e-2)
     dev = api.synth_device(kind, spec)
     try:
@@ -45,7 +46,7 @@ def test_ft_batch_chunks_match_alone():
     """A batch of (fT) systems with more than 32768 items in all runs as
     several grids; each system's energy still equals its own plan's."""
     kind = "rft"
-    dims_list = [(10, 128), (5, 30), (4, 37)]
+    dims_list = [(12, 160), (5, 30), (4, 37)]
     systems = []
     try:
         for n, d in enumerate(dims_list):
