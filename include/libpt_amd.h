@@ -49,6 +49,24 @@ struct libpt_amd_stats {
 double libpt_amd_energy(int call, int form, size_t oa, size_t va, size_t ob,
     size_t vb, const void *const *arrays, struct libpt_amd_stats *stats);
 
+/* Allocation-free variant of libpt_amd_energy.  No counterpart in the
+ * reference libpt: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The inputs are device pointers in pt.h order
+ * (dev_arrays, read only); the work runs on `stream` (a hipStream_t, NULL
+ * for the null stream), which the call synchronises; the energy is
+ * returned, NaN on error (libpt_amd_last_error says why).  The call
+ * allocates nothing: every buffer it needs (packed and split operands,
+ * work lists, per-item energies, (fT) save areas) comes from one static
+ * device array of LIBPT_AMD_NOALLOC_SCRATCH bytes per GPU, which each GPU
+ * the library runs on holds until the process ends.  A call whose
+ * buffers do not fit is rejected with an error (FP64 restricted (T) fits
+ * up to about o=10, v=120).  Calls on one GPU share the array: overlapping
+ * calls would race on it, so the library runs them one at a time. */
+#define LIBPT_AMD_NOALLOC_SCRATCH ((size_t)256 << 20)
+double libpt_amd_energy_noalloc(int call, int form, size_t oa, size_t va,
+    size_t ob, size_t vb, const void *const *dev_arrays, void *stream);
+
 /* ---- plans over device-resident inputs ---- */
 typedef struct libpt_amd_plan libpt_amd_plan;
 

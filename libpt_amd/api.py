@@ -71,6 +71,9 @@ def lib():
         L.libpt_amd_energy.restype = _d
         L.libpt_amd_energy.argtypes = [_i, _i, _sz, _sz, _sz, _sz,
                                        C.POINTER(_p), C.POINTER(Stats)]
+        L.libpt_amd_energy_noalloc.restype = _d
+        L.libpt_amd_energy_noalloc.argtypes = [_i, _i, _sz, _sz, _sz, _sz,
+                                               _p, _p]
         L.libpt_amd_plan_create.restype = _p
         L.libpt_amd_plan_create.argtypes = [_i, _i, _sz, _sz, _sz, _sz,
                                             C.POINTER(_p)]
@@ -218,6 +221,19 @@ def energy(kind: str, form, dims, arrays, with_stats=False):
     if math.isnan(e):
         raise LibptError(last_error())
     return (e, st) if with_stats else e
+
+
+def energy_noalloc(kind: str, form, dims, dev: "DeviceArrays",
+                   stream=None) -> float:
+    """libpt_amd_energy_noalloc: device-resident inputs, every buffer from
+    the library's static scratch (no device allocation); raises on error."""
+    form = FORMS.get(form, form)
+    arr = (_p * len(dev.ptrs))(*dev.ptrs)
+    e = lib().libpt_amd_energy_noalloc(CALLS[kind], form, *_dims4(kind, dims),
+                                       arr, stream)
+    if math.isnan(e):
+        raise LibptError(last_error())
+    return e
 
 
 @dataclass

@@ -23,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(ROOT, "_build", "obj")
 LIB = os.path.join(HERE, "libpt_amd.so")
 
-SOURCES = ["pt_kernel.hip", "pt_host.cpp", "pt_terms.cpp"]
+SOURCES = ["pt_kernel.hip", "pt_noalloc.hip", "pt_host.cpp", "pt_terms.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

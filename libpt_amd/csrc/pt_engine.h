@@ -187,6 +187,16 @@ int pt_launch_copy(const double *src, double *dst, int64_t n, void *stream);
 int pt_launch_synth(const void *spec, int tensor, void *dst, int64_t n,
     int fp32, void *stream);
 
+
+/* host engine (pt_host.cpp) for pt_noalloc.hip: the device every call
+ * runs on (made current; -1 on error), the calling thread's error text
+ * (NULL clears it), and one call whose device buffers all come from
+ * [scratch, scratch + cap) */
+int pt_device_init(void);
+void pt_set_error(const char *msg);
+double pt_noalloc_energy(int call, int form, size_t oa, size_t va,
+    size_t ob, size_t vb, const void *const *dev_arrays, void *scratch,
+    size_t cap, void *stream);
 #ifdef __cplusplus
 }
 #endif

@@ -1638,6 +1638,55 @@ libpt_amd_last_error(void)
 	return g_err;
 }
 
+extern "C" int
+pt_device_init(void)
+{
+	return dev_init() ? g_dev : -1;
+}
+
+extern "C" void
+pt_set_error(const char *msg)
+{
+	if (msg)
+		set_err("%s", msg);
+	else
+		g_err[0] = 0;
+}
+
+/*
+ * The body of libpt_amd_energy_noalloc (pt_noalloc.hip): one call on
+ * device-resident inputs whose every device buffer -- packed operands,
+ * split operands, term tables, work lists, per-item energies, (fT) save
+ * areas -- comes from the static scratch [scratch, scratch + cap); the plan
+ * itself lives on this stack frame and its lists go up through a stack
+ * buffer.  Runs on the caller's stream and synchronises it.
+ */
+extern "C" double
+pt_noalloc_energy(int call, int form, size_t oa, size_t va, size_t ob,
+    size_t vb, const void *const *dev_arrays, void *scratch, size_t cap,
+    void *stream)
+{
+	if (!dev_init())
+		return NAN;
+	Arena ar;
+	ar.base = (char *)scratch;
+	ar.cap = cap;
+	libpt_amd_plan pl;
+	pl.arena = &ar;
+	pl.own_stream = false;
+	pl.stream = (hipStream_t)stream;
+	double e = NAN;
+	if (plan_init(&pl, call, form, oa, va, ob, vb, dev_arrays, true)) {
+		bool ok = true;
+		for (int sc = 0; sc < pl.cases.size() && ok; ++sc)
+			ok = libpt_amd_plan_run(&pl, sc, 0, pl.cases[sc].n_ijk) == 0;
+		if (ok)
+			e = libpt_amd_plan_energy(&pl);
+	}
+	plan_release(&pl);
+	return e;
+}
+
 namespace {
 
 /*
