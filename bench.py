@@ -148,22 +148,33 @@ def run_form(api, form, dims, dev, args, nbatch):
 def run_batch(api, form, n, seed, tune=0, sample=8):
     """n independent restricted systems (o in [4, 20], v in [40, 160]) in
     one graph launch; one untimed replay first.  tune: autotune every
-    system's tile order first (each distinct size timed once); the plans
-    timed one after another then take the same orders."""
+    system's tile geometry and order first (each distinct size timed once);
+    the plans timed one after another then take the same choices."""
     dims = api.batch_dims(n, seed=seed)
     systems = [(d, api.synth_device("rpt", api.make_spec("rpt", d,
                                                          seed=seed + 1 + k),
                                     fp32=form == "fp32"))
                for k, d in enumerate(dims)]
     try:
+        timed, tune_s = 0, 0.0
+        if tune:
+            # the geometry of a batch's plans is fixed when its graph is
+            # recorded: tune each distinct size on a plan of its own first
+            # (geometry and order go to the tuning cache, which the
+            # batch's plans read)
+            t0 = time.perf_counter()
+            seen = set()
+            for d, dev in systems:
+                if d in seen:
+                    continue
+                seen.add(d)
+                with api.Plan("rpt", form, d, dev) as pl:
+                    pl.tune(sample)
+                timed += 1
+            tune_s = time.perf_counter() - t0
         t0 = time.perf_counter()
         b = api.Batch("rpt", form, systems)
         prep = time.perf_counter() - t0
-        timed, tune_s = 0, 0.0
-        if tune:
-            t0 = time.perf_counter()
-            timed = b.tune(sample)
-            tune_s = time.perf_counter() - t0
         with b:
             b.run()
             b.energies()

@@ -85,10 +85,11 @@ double libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *plan, int sc);
 /* multiplier of spin case sc in the call's energy (2 for restricted) */
 double libpt_amd_plan_factor(const libpt_amd_plan *plan, int sc);
 /* tile-kernel geometry of the plan: out3 = (waves per workgroup, c tile
- * width, ijk per item group).  Defaults: 8 waves (32 x 32 x 32 tiles) for
- * FP64 and the split forms, 4 waves (32 x 32 x 32 with 32 c per wave) for
- * FP32; no grouping of consecutive ijk (1).  LIBPT_AMD_WAVES=4|8 and
- * LIBPT_AMD_KGROUP=n override them for new plans. */
+ * width, ijk per item group).  Default: 4 waves (32 x 32 x 16 tiles for
+ * FP64 and the split forms, 32 x 32 x 32 for FP32), unless the tuning
+ * cache holds a geometry for the plan's sizes; no grouping of consecutive
+ * ijk (1).  LIBPT_AMD_WAVES=4|8 and LIBPT_AMD_KGROUP=n override them for
+ * new plans. */
 int libpt_amd_plan_geometry(const libpt_amd_plan *plan, int *out3);
 
 /* enqueue ijk items [ijk_begin, ijk_end) of spin case sc on the plan's
@@ -104,16 +105,27 @@ int libpt_amd_plan_ijk_energies(libpt_amd_plan *plan, int sc, double *out);
 /* the ijk triples of spin case sc, 3 * n_ijk ints, in work-list order */
 int libpt_amd_plan_ijk_list(const libpt_amd_plan *plan, int sc, int32_t *out);
 
-/* Tile order of the work list: tiles (A, B, C) are visited in blocks of
- * ga x gb x gc so that workgroups running together share operand slices
- * in L2.  set_block applies one blocking; tune times candidate blockings
- * on the first `sample_ijk` ijk triples of each spin case (0: 8), applies
- * the fastest and records it for plans of the same (form, call, sizes),
- * also in the file named by LIBPT_AMD_TUNE_CACHE when set; out3 (may be
- * NULL) receives (ga, gb, gc).  LIBPT_AMD_TILE_BLOCK="ga,gb,gc" forces a
+/* Tile geometry and order of the work list.  Tiles (A, B, C) are visited
+ * in blocks of ga x gb x gc so that workgroups running together share
+ * operand slices in L2; set_block applies one blocking.  set_geometry
+ * switches the tile kernel between 4 waves (32 x 32 x 16 tiles in FP64
+ * and the split forms, 32 x 32 x 32 in FP32) and 8 waves (32 x 32 x 32
+ * tiles), rebuilding the plan's tile lists (not for the plans of a
+ * batch).  tune times both geometries (only the plan's own when
+ * LIBPT_AMD_WAVES is set) and the candidate blockings on the first
+ * `sample_ijk` ijk triples of each spin case (0: 8), applies the fastest
+ * and records it for plans of the same (form, call, sizes), also in the
+ * file named by LIBPT_AMD_TUNE_CACHE when set; out4 (may be NULL)
+ * receives (ga, gb, gc, waves).  LIBPT_AMD_TILE_BLOCK="ga,gb,gc" forces a
  * blocking for new plans.  0 on success. */
 int libpt_amd_plan_set_block(libpt_amd_plan *plan, int ga, int gb, int gc);
-int libpt_amd_plan_tune(libpt_amd_plan *plan, int sample_ijk, int *out3);
+int libpt_amd_plan_set_geometry(libpt_amd_plan *plan, int waves);
+int libpt_amd_plan_tune(libpt_amd_plan *plan, int sample_ijk, int *out4);
+/* records a geometry (waves 4 or 8; 0: the default) and blocking for new
+ * plans of (call, form, sizes), as tune does after timing, e.g. from a
+ * tuning table kept by the caller; 0 on success */
+int libpt_amd_tune_record(int call, int form, size_t oa, size_t va,
+    size_t ob, size_t vb, int ga, int gb, int gc, int waves);
 
 /* GPU-timed window on the plan's stream: begin/end bracket enqueued work */
 int libpt_amd_plan_timer_begin(libpt_amd_plan *plan);

@@ -101,6 +101,11 @@ def lib():
         L.libpt_amd_plan_ijk_list.argtypes = [_p, _i, _p]
         L.libpt_amd_plan_set_block.restype = _i
         L.libpt_amd_plan_set_block.argtypes = [_p, _i, _i, _i]
+        L.libpt_amd_tune_record.restype = _i
+        L.libpt_amd_tune_record.argtypes = [_i, _i, _sz, _sz, _sz, _sz, _i,
+                                            _i, _i, _i]
+        L.libpt_amd_plan_set_geometry.restype = _i
+        L.libpt_amd_plan_set_geometry.argtypes = [_p, _i]
         L.libpt_amd_plan_tune.restype = _i
         L.libpt_amd_plan_tune.argtypes = [_p, _i, _p]
         L.libpt_amd_plan_timer_begin.restype = _i
@@ -234,6 +239,15 @@ def energy_noalloc(kind: str, form, dims, dev: "DeviceArrays",
     if math.isnan(e):
         raise LibptError(last_error())
     return e
+
+
+def tune_record(kind: str, form, dims, block, waves=0):
+    """libpt_amd_tune_record: geometry (waves 4 or 8, 0 for the default)
+    and blocking (ga, gb, gc) for new plans of this call, form and size."""
+    form = FORMS.get(form, form)
+    if lib().libpt_amd_tune_record(CALLS[kind], form, *_dims4(kind, dims),
+                                   *block, waves) != 0:
+        raise LibptError(last_error())
 
 
 @dataclass
@@ -397,13 +411,18 @@ class Plan:
         if lib().libpt_amd_plan_set_block(self.h, ga, gb, gc) != 0:
             raise LibptError(last_error())
 
+    def set_geometry(self, waves):
+        if lib().libpt_amd_plan_set_geometry(self.h, waves) != 0:
+            raise LibptError(last_error())
+
     def tune(self, sample_ijk=8):
-        """Autotune the tile order (see libpt_amd_plan_tune); returns the
-        chosen (ga, gb, gc)."""
-        out = (C.c_int * 3)()
+        """Autotune the tile geometry and order (see libpt_amd_plan_tune);
+        returns the chosen blocking (ga, gb, gc); geometry() gives the
+        chosen waves."""
+        out = (C.c_int * 4)()
         if lib().libpt_amd_plan_tune(self.h, sample_ijk, out) != 0:
             raise LibptError(last_error())
-        return tuple(out)
+        return tuple(out[:3])
 
     def timer_begin(self):
         lib().libpt_amd_plan_timer_begin(self.h)

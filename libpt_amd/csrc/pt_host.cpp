@@ -163,9 +163,11 @@ flop_mixed(double oa, double va, double ob, double vb)
 	    4 * ob * va * va * vb;
 }
 
-/* blocking of the tile order (A, B, C tiles per block) */
+/* blocking of the tile order (A, B, C tiles per block) and, when the
+ * tuner recorded one, the tile geometry (waves per workgroup; 0: none) */
 struct TileBlock {
 	int ga = 8, gb = 8, gc = 2;
+	int nw = 0;
 };
 
 /* LIBPT_AMD_TILE_BLOCK="ga,gb,gc" forces a blocking */
@@ -185,11 +187,12 @@ tile_block_env(TileBlock &tb)
 }
 
 /*
- * Tuning cache: the blocking chosen by libpt_amd_plan_tune per (form,
- * call, oa, va, ob, vb), kept for the process and, when
- * LIBPT_AMD_TUNE_CACHE names a file, read from and appended to it (one
- * line per entry: form call oa va ob vb ga gb gc).  Lookups and stores
- * hold g_mu.
+ * Tuning cache: the tile geometry and blocking chosen by
+ * libpt_amd_plan_tune per (form, call, oa, va, ob, vb), kept for the
+ * process and, when LIBPT_AMD_TUNE_CACHE names a file, read from and
+ * appended to it (one line per entry: form call oa va ob vb ga gb gc nw;
+ * lines without nw keep the default geometry).  Lookups and stores hold
+ * g_mu.
  */
 struct TuneEntry {
 	int form, call;
@@ -210,10 +213,17 @@ tune_load()
 	FILE *f = path ? fopen(path, "r") : nullptr;
 	if (!f)
 		return;
-	TuneEntry t;
-	long long oa, va, ob, vb;
-	while (fscanf(f, "%d %d %lld %lld %lld %lld %d %d %d", &t.form, &t.call,
-	    &oa, &va, &ob, &vb, &t.tb.ga, &t.tb.gb, &t.tb.gc) == 9) {
+	char line[256];
+	while (fgets(line, sizeof(line), f)) {
+		TuneEntry t;
+		long long oa, va, ob, vb;
+		const int n = sscanf(line, "%d %d %lld %lld %lld %lld %d %d %d %d",
+		    &t.form, &t.call, &oa, &va, &ob, &vb, &t.tb.ga, &t.tb.gb,
+		    &t.tb.gc, &t.tb.nw);
+		if (n < 9)
+			continue;
+		if (n == 9 || (t.tb.nw != 4 && t.tb.nw != 8))
+			t.tb.nw = 0;
 		t.oa = oa;
 		t.va = va;
 		t.ob = ob;
@@ -255,9 +265,9 @@ tune_store(int form, int call, int64_t oa, int64_t va, int64_t ob,
 	const char *path = getenv("LIBPT_AMD_TUNE_CACHE");
 	FILE *f = path ? fopen(path, "a") : nullptr;
 	if (f) {
-		fprintf(f, "%d %d %lld %lld %lld %lld %d %d %d\n", form, call,
+		fprintf(f, "%d %d %lld %lld %lld %lld %d %d %d %d\n", form, call,
 		    (long long)oa, (long long)va, (long long)ob, (long long)vb,
-		    tb.ga, tb.gb, tb.gc);
+		    tb.ga, tb.gb, tb.gc, tb.nw);
 		fclose(f);
 	}
 }
@@ -293,6 +303,9 @@ struct libpt_amd_plan {
 	int call = 0, form = 0;
 	int nw = 4;                 /* waves per workgroup of the tile kernel */
 	int kgrp = 1;               /* ijk per item group of a launch */
+	bool vct = true;            /* VC packed a second time, transposed */
+	bool fixed_geometry = false; /* a batch's plan: its launches are
+	                                recorded, so the tuner keeps nw */
 	int64_t oa = 0, va = 0, ob = 0, vb = 0;
 	hipStream_t stream = nullptr;
 	bool own_stream = true;     /* created (and destroyed) by the plan */
@@ -381,6 +394,21 @@ plan_alloc(libpt_amd_plan *pl, void **p, size_t bytes)
 		return false;
 	pl->owned.push_back(*p);
 	return true;
+}
+
+/* frees one of the plan's hipMalloc buffers (no-op for null) */
+void
+plan_drop(libpt_amd_plan *pl, void *p)
+{
+	if (!p || pl->arena)
+		return;
+	for (int i = 0; i < pl->owned.size(); ++i)
+		if (pl->owned[i] == p) {
+			(void)hipFree(p);
+			pl->owned[i] = pl->owned[pl->owned.size() - 1];
+			pl->owned.n--;
+			return;
+		}
 }
 
 /* Host lists go to the device in chunks of N triples of ints through one
@@ -486,9 +514,35 @@ case_set_block(const libpt_amd_plan *pl, Case &c, const TileBlock &tb)
 	return true;
 }
 
-/* work list of one spin case: ijk items x (A, B, C) tiles.  A >= B; for
- * the same-spin case (a > b > c) only c tiles that reach below the top of
- * the B block. */
+/* tile list of one spin case for c tiles of width tcw, and its item
+ * energy slots (n_ijk x tiles, cleared).  A >= B; for the same-spin case
+ * (a > b > c) only c tiles that reach below the top of the B block. */
+bool
+case_tiles(libpt_amd_plan *pl, Case &c, int64_t nc, int tcw)
+{
+	c.nC = (int)((nc + tcw - 1) / tcw);
+	c.tcw = tcw;
+	int nt = 0;
+	(void)tile_order(c, c.block, [&](int, int, int) { ++nt; });
+	c.n_tiles = nt;
+	if (!plan_alloc(pl, (void **)&c.d_tiles, 3 * sizeof(int32_t) *
+	    (size_t)nt) ||
+	    !plan_alloc(pl, (void **)&c.d_partial,
+	    sizeof(double) * (size_t)(c.n_ijk * nt)))
+		return false;
+	ListBuf lb;
+	if (!lb.get(pl))
+		return false;
+	ListUpload ut(c.d_tiles, lb.p);
+	(void)tile_order(c, c.block, [&](int a, int b, int x) { ut.put(a, b, x); });
+	if (!ut.flush())
+		return false;
+	HCHK(hipMemset(c.d_partial, 0, sizeof(double) * (size_t)(c.n_ijk * nt)),
+	    false);
+	return true;
+}
+
+/* work list of one spin case: ijk items x (A, B, C) tiles */
 bool
 case_lists(libpt_amd_plan *pl, Case &c, int same, int64_t o1, int64_t o2,
     int64_t na, int64_t nc, int tcw)
@@ -496,18 +550,9 @@ case_lists(libpt_amd_plan *pl, Case &c, int same, int64_t o1, int64_t o2,
 	int64_t nijk = same ? o1 * (o1 - 1) * (o1 - 2) / 6 : o1 * (o1 - 1) / 2 * o2;
 	c.same = same;
 	c.nA = (int)((na + PT_TA - 1) / PT_TA);
-	c.nC = (int)((nc + tcw - 1) / tcw);
-	c.tcw = tcw;
-	int nt = 0;
-	(void)tile_order(c, c.block, [&](int, int, int) { ++nt; });
 	c.n_ijk = nijk;
-	c.n_tiles = nt;
 	if (!plan_alloc(pl, (void **)&c.d_ijk, 3 * sizeof(int32_t) *
-	    (size_t)nijk) ||
-	    !plan_alloc(pl, (void **)&c.d_tiles, 3 * sizeof(int32_t) *
-	    (size_t)nt) ||
-	    !plan_alloc(pl, (void **)&c.d_partial,
-	    sizeof(double) * (size_t)(nijk * nt)))
+	    (size_t)nijk))
 		return false;
 	ListBuf lb;
 	if (!lb.get(pl))
@@ -519,13 +564,7 @@ case_lists(libpt_amd_plan *pl, Case &c, int same, int64_t o1, int64_t o2,
 				ui.put((int)i, (int)j, (int)k);
 	if (!ui.flush())
 		return false;
-	ListUpload ut(c.d_tiles, lb.p);
-	(void)tile_order(c, c.block, [&](int a, int b, int x) { ut.put(a, b, x); });
-	if (!ut.flush())
-		return false;
-	HCHK(hipMemset(c.d_partial, 0, sizeof(double) * (size_t)(nijk * nt)),
-	    false);
-	return true;
+	return case_tiles(pl, c, nc, tcw);
 }
 
 bool
@@ -548,7 +587,7 @@ case_finish(libpt_amd_plan *pl, Case &c, int form, int kind, int nset,
 		if (kind == PT_AAA)
 			pt_terms_same(c.terms, s, oa, va);
 		else
-			pt_terms_mixed(c.terms, s, oa, va, ob, vb);
+			pt_terms_mixed(c.terms, s, oa, va, ob, vb, pl->vct);
 	}
 	/* Q slab offsets: API elements (K q_k per slab) -> bytes of the
 	 * packed copy (pt_kgroups q_k groups per slab) */
@@ -627,6 +666,51 @@ plan_wsave(libpt_amd_plan *pl)
 	return true;
 }
 
+/* A new tile geometry (waves per workgroup: 4 or 8) for a plan: the tile
+ * lists, item slots and save areas are rebuilt for the new c tile width;
+ * operands, terms and ijk lists stay.  Not for plans in a batch (their
+ * launches are recorded in the batch's graph) or on an arena. */
+bool
+plan_set_geometry(libpt_amd_plan *pl, int nw)
+{
+	if (nw != 4 && nw != 8) {
+		set_err("waves per workgroup must be 4 or 8, not %d", nw);
+		return false;
+	}
+	if (nw == pl->nw)
+		return true;
+	if (pl->fixed_geometry || pl->arena) {
+		set_err("the geometry of this plan is fixed");
+		return false;
+	}
+	HCHK(hipStreamSynchronize(pl->stream), false);
+	const bool had_wsave = pl->wsave != nullptr;
+	plan_drop(pl, pl->wsave);
+	pl->wsave = nullptr;
+	pl->wslots = 0;
+	pl->nw = nw;
+	for (Case &c : pl->cases) {
+		if (c.n_ijk == 0)
+			continue;
+		pt_launch &L = c.L;
+		plan_drop(pl, c.d_tiles);
+		plan_drop(pl, c.d_partial);
+		c.d_tiles = nullptr;
+		c.d_partial = nullptr;
+		L.nw = nw;
+		L.tcw = pt_tcw(pl->form, L.nset, nw);
+		if (!case_tiles(pl, c, L.nc, L.tcw))
+			return false;
+		L.tiles = c.d_tiles;
+		L.n_tiles = c.n_tiles;
+		L.partial = c.d_partial;
+	}
+	if (had_wsave && !plan_wsave(pl))
+		return false;
+	HCHK(hipDeviceSynchronize(), false);
+	return true;
+}
+
 /* Stages the operands and work lists of one call into pl, which comes
  * with its arena (or none) and, when pl->own_stream is false, its stream.
  * own_wsave: allocate the (fT) save areas now, so that runs allocate
@@ -650,6 +734,8 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 		pl->nw = atoi(w) == 4 ? 4 : atoi(w) == 8 ? 8 : pl->nw;
 	if (const char *g = getenv("LIBPT_AMD_KGROUP"))
 		pl->kgrp = std::max(1, std::min(64, atoi(g)));
+	if (const char *t = getenv("LIBPT_AMD_VC_TRANSPOSED"))
+		pl->vct = atoi(t) != 0;
 	pl->call = call;
 	pl->form = form;
 	pl->oa = oa;
@@ -767,7 +853,7 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 				PFAIL(plan_alloc(pl, &dst, (size_t)(g.R * nkb * g.PL) * GB));
 				const char *src = (const char *)arr[api] + esz * (size_t)g.off;
 				PFAIL(pt_launch_pack(src, dst, g.R, g.K, nkb, g.PL, KC, form,
-				    sc, pl->stream) == 0 ||
+				    sc, 0, pl->stream) == 0 ||
 				    (set_err("pack launch failed"), false));
 				dstb.p[b] = (const char *)dst;
 			}
@@ -776,6 +862,28 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 				dstb.p[3] = dstb.p[1];
 			}
 		}
+	}
+	/* the mixed ovvv blocks as the mixed-spin case reads its VC operand
+	 * (VC[k, d, c, b] with (u, w) = (b or a, c)): packed again with the
+	 * plane transposed, so that the 32 w of a DMA row are contiguous
+	 * (abab plane [va][vb], baba [vb][va]) */
+	Blocks qVt[2];
+	for (int s = 0; s < nset_ && pl->vct; ++s) {
+		const double sc = pl->scale[2 * s + 1];
+		for (int b = 1; b < (restricted ? 2 : 4); b += 2) {
+			const Geo g = geo(1, b);
+			const int64_t nkb = pt_kgroups(form, g.K);
+			void *dst;
+			PFAIL(plan_alloc(pl, &dst, (size_t)(g.R * nkb * g.PL) * GB));
+			const char *src = (const char *)arr[famV[s]] + esz *
+			    (size_t)g.off;
+			PFAIL(pt_launch_pack(src, dst, g.R, g.K, nkb, g.PL, KC, form, sc,
+			    b == 1 ? va : vb, pl->stream) == 0 ||
+			    (set_err("pack launch failed"), false));
+			qVt[s].p[b] = (const char *)dst;
+		}
+		if (restricted)
+			qVt[s].p[3] = qVt[s].p[1];
 	}
 	PFAIL(hipStreamSynchronize(pl->stream) == hipSuccess ||
 	    (set_err("operand preparation failed"), false));
@@ -846,6 +954,8 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 	TileBlock blk;
 	if (!tile_block_env(blk))
 		(void)tune_lookup(form, call, oa, va, ob, vb, blk, !pl->arena);
+	if (blk.nw && !getenv("LIBPT_AMD_WAVES"))
+		pl->nw = blk.nw;
 	for (const CaseSpec &cs : specs) {
 		Case c;
 		c.block = blk;
@@ -867,7 +977,7 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 			c.L.buf[sl + 5] = opO[s].p[mixC];
 			c.L.buf[sl + 6] = qV[s].p[same];
 			c.L.buf[sl + 7] = qV[s].p[mixB];
-			c.L.buf[sl + 8] = qV[s].p[mixC];
+			c.L.buf[sl + 8] = pl->vct ? qVt[s].p[mixC] : qV[s].p[mixC];
 			c.L.buf[sl + 9] = qT[s].p[same];
 			c.L.buf[sl + 10] = qT[s].p[mixB];
 			c.L.buf[sl + 11] = qT[s].p[mixC];
@@ -1171,15 +1281,50 @@ libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
 	return 0;
 }
 
+extern "C" int
+libpt_amd_plan_set_geometry(libpt_amd_plan *pl, int nw)
+{
+	if (!dev_init())
+		return -1;
+	if (!pl) {
+		set_err("null plan");
+		return -1;
+	}
+	return plan_set_geometry(pl, nw) ? 0 : -1;
+}
+
+extern "C" int
+libpt_amd_tune_record(int call, int form, size_t oa, size_t va, size_t ob,
+    size_t vb, int ga, int gb, int gc, int waves)
+{
+	if (call < 0 || call > 3 || form < 0 || form >= PT_NFORM || ga <= 0 ||
+	    gb <= 0 || gc <= 0 || (waves != 0 && waves != 4 && waves != 8)) {
+		set_err("bad tuning record");
+		return -1;
+	}
+	const bool restricted = call == LIBPT_AMD_RPT || call == LIBPT_AMD_RFT;
+	TileBlock tb;
+	tb.ga = ga;
+	tb.gb = gb;
+	tb.gc = gc;
+	tb.nw = waves;
+	tune_store(form, call, (int64_t)oa, (int64_t)va,
+	    (int64_t)(restricted ? oa : ob), (int64_t)(restricted ? va : vb), tb);
+	return 0;
+}
+
 /*
- * Autotuner of the tile order: for each candidate blocking, time the first
- * `sample` ijk triples of every spin case (after one untimed pass), weight
- * each case by its ijk count, keep the fastest, and record it in the
- * tuning cache.  The energies of the sampled items are overwritten by the
- * next run of the plan.
+ * Autotuner of the tile geometry and order: for each tile geometry (4
+ * waves with 16-wide c tiles in FP64 and the split forms, 8 waves with
+ * 32-wide c tiles; only the plan's own for a plan in a batch) and each
+ * candidate blocking, time the first `sample` ijk triples of every spin
+ * case (after one untimed pass per geometry), weight each case by its ijk
+ * count, keep the fastest, and record it in the tuning cache.  The
+ * energies of the sampled items are overwritten by the next run of the
+ * plan.  out4 (optional): ga, gb, gc, waves.
  */
 extern "C" int
-libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out3)
+libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out4)
 {
 	if (!dev_init())
 		return -1;
@@ -1194,43 +1339,56 @@ libpt_amd_plan_tune(libpt_amd_plan *pl, int sample, int *out3)
 	const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
 	if (sample <= 0)
 		sample = 8;
+	const bool geo = !pl->fixed_geometry && !pl->arena &&
+	    !getenv("LIBPT_AMD_WAVES");
+	int nws[2] = {pl->nw, pl->nw == 8 ? 4 : 8};
+	const int ngeo = geo ? 2 : 1;
 	double best = 0;
-	int bi = -1;
-	for (int ci = -1; ci < ncand; ++ci) {
-		/* ci = -1: one untimed warm pass with the first candidate */
-		const int *cb = cand[ci < 0 ? 0 : ci];
-		if (libpt_amd_plan_set_block(pl, cb[0], cb[1], cb[2]) != 0)
+	int bi = -1, bw = pl->nw;
+	for (int gi = 0; gi < ngeo; ++gi) {
+		if (!plan_set_geometry(pl, nws[gi]))
 			return -1;
-		double t = 0;
-		for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
-			const Case &c = pl->cases[sc];
-			if (c.n_ijk == 0)
-				continue;
-			const int64_t m = c.n_ijk < sample ? c.n_ijk : sample;
-			if (libpt_amd_plan_timer_begin(pl) != 0 ||
-			    libpt_amd_plan_run(pl, sc, 0, m) != 0)
+		for (int ci = -1; ci < ncand; ++ci) {
+			/* ci = -1: one untimed warm pass with the first
+			 * candidate */
+			const int *cb = cand[ci < 0 ? 0 : ci];
+			if (libpt_amd_plan_set_block(pl, cb[0], cb[1], cb[2]) != 0)
 				return -1;
-			const double ms = libpt_amd_plan_timer_end_ms(pl);
-			if (isnan(ms))
-				return -1;
-			t += ms * (double)c.n_ijk / (double)m;
-		}
-		if (ci >= 0 && (bi < 0 || t < best)) {
-			best = t;
-			bi = ci;
+			double t = 0;
+			for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
+				const Case &c = pl->cases[sc];
+				if (c.n_ijk == 0)
+					continue;
+				const int64_t m = c.n_ijk < sample ? c.n_ijk : sample;
+				if (libpt_amd_plan_timer_begin(pl) != 0 ||
+				    libpt_amd_plan_run(pl, sc, 0, m) != 0)
+					return -1;
+				const double ms = libpt_amd_plan_timer_end_ms(pl);
+				if (isnan(ms))
+					return -1;
+				t += ms * (double)c.n_ijk / (double)m;
+			}
+			if (ci >= 0 && (bi < 0 || t < best)) {
+				best = t;
+				bi = ci;
+				bw = nws[gi];
+			}
 		}
 	}
-	if (libpt_amd_plan_set_block(pl, cand[bi][0], cand[bi][1], cand[bi][2]))
+	if (!plan_set_geometry(pl, bw) ||
+	    libpt_amd_plan_set_block(pl, cand[bi][0], cand[bi][1], cand[bi][2]))
 		return -1;
 	TileBlock tb;
 	tb.ga = cand[bi][0];
 	tb.gb = cand[bi][1];
 	tb.gc = cand[bi][2];
+	tb.nw = geo ? bw : 0;
 	tune_store(pl->form, pl->call, pl->oa, pl->va, pl->ob, pl->vb, tb);
-	if (out3) {
-		out3[0] = tb.ga;
-		out3[1] = tb.gb;
-		out3[2] = tb.gc;
+	if (out4) {
+		out4[0] = tb.ga;
+		out4[1] = tb.gb;
+		out4[2] = tb.gc;
+		out4[3] = bw;
 	}
 	return 0;
 }
@@ -1336,6 +1494,7 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 		    (size_t)d[1], (size_t)d[2], (size_t)d[3], dev_arrays + na * s,
 		    false);
 		BFAIL(pl != nullptr);
+		pl->fixed_geometry = true;
 		b->plans.push_back(pl);
 		for (int sc = 0; sc < (int)pl->cases.size(); ++sc) {
 			Case &c = pl->cases[sc];
@@ -1373,13 +1532,16 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 	BFAIL(dev_alloc((void **)&b->d_segs, sizeof(pt_launch) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_begin, sizeof(int64_t) * (nseg + 1)));
 	BFAIL(dev_alloc((void **)&b->d_out, sizeof(double) * (nseg + 1)));
-	const int nw = b->plans.empty() ? 4 : b->plans[0]->nw;
-	const int64_t wbytes = pt_wsave_bytes(form, 2, nw);
+	/* systems may differ in geometry (recorded by the tuner): save
+	 * areas of the largest */
+	int64_t wbytes = 1;
+	for (const libpt_amd_plan *pl : b->plans)
+		wbytes = std::max(wbytes, pt_wsave_bytes(form, 2, pl->nw));
 	if (b->nset == 2 && b->total > 0) {
 		int64_t most = 1;
 		for (const Seg &g : segs)
 			most = std::max(most, g.items);
-		b->wslots = std::min<int64_t>(pt_ft_chunk(form, nw), most);
+		b->wslots = std::min<int64_t>(((int64_t)4 << 30) / wbytes, most);
 		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots * (size_t)wbytes));
 	}
 	if (nseg > 0) {

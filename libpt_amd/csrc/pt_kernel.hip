@@ -2,9 +2,9 @@
  * CDNA4 (gfx950) kernels of the perturbative-triples engine.
  *
  * k_tiles: one workgroup = one ijk item x one (A, B, C) tile of 32 a x 32 b
- * x TCW c (TCW = 32 for (T), 16 for (fT)).  Four waves each own 16 x 16 x TCW
- * (a, b, c) block of the connected triples W in MFMA accumulators and never
- * write W to memory:
+ * x TC c.  Its NW waves (4 or 8; Geo below) each own a 16 a x 16 b x CW c
+ * block of the connected triples W in MFMA accumulators and never write W
+ * to memory:
  *
  *   phase 1 -- accumulator layout [a][b]@c (one 16x16 MFMA block per c):
  *       terms whose single index is a  (A = S rows a,  B = Q[.][b][c])
@@ -17,8 +17,10 @@
  *       ((fT): (W_L - Z_L) W_R / Dn), compensated FP64 sum per lane,
  *       one deterministic reduction per workgroup into partial[item].
  *
- * Operand tiles are staged global -> LDS per kappa chunk with the
- * contiguous global index mapped to consecutive threads.
+ * Operands go from HBM straight into LDS with buffer-load-to-LDS DMAs (Q
+ * from the packed copies in 16-byte runs, S rows in dwords), several steps
+ * ahead of the step computed; waves meet at fence-free barriers and wait
+ * only for the DMAs of the step they read next.
  *
  * Forms: FP64 (v_mfma_f64_16x16x4_f64), FP32 (v_mfma_f32_16x16x4_f32),
  * split BF16 / FP16 (hi+lo pairs, hi*hi + hi*lo + lo*hi packed into the
@@ -1206,16 +1208,19 @@ split2(double x, int fp16, uint16_t &hb, uint16_t &lb)
 }
 
 /* [R][K][PL] -> [R][nkb][PL][KC] (FP64/FP32) or [R][nkb][PL][hi KC][lo KC]
- * (split, of x * scale); kappa >= K is zero.  One thread per group: reads
- * coalesce over PL, each group is written contiguously. */
+ * (split, of x * scale); kappa >= K is zero.  P1 > 0: the plane PL = P1 x
+ * P2 ([x][y], y contiguous) is stored transposed, [y][x] with x
+ * contiguous.  One thread per group, each group written contiguously. */
 __global__ void
 k_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
-    int64_t PL, int KC, int mode, double scale)
+    int64_t PL, int KC, int mode, double scale, int64_t P1)
 {
 	const int64_t total = R * nkb * PL;
 	for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	    t < total; t += (int64_t)gridDim.x * blockDim.x) {
-		const int64_t pl = t % PL, rk = t / PL, kb = rk % nkb, r = rk / nkb;
+		const int64_t pd = t % PL, rk = t / PL, kb = rk % nkb, r = rk / nkb;
+		/* destination plane index pd -> source plane index pl */
+		const int64_t pl = P1 > 0 ? (pd % P1) * (PL / P1) + pd / P1 : pd;
 		const int64_t k0 = kb * KC;
 		const int64_t sbase = (r * K + k0) * PL + pl;
 		if (mode == PT_FORM_FP64) {
@@ -1396,14 +1401,14 @@ pt_launch_split(const double *src, uint32_t *dst, int64_t n, int fp16,
 
 extern "C" int
 pt_launch_pack(const void *src, void *dst, int64_t R, int64_t K, int64_t nkb,
-    int64_t PL, int KC, int mode, double scale, void *stream)
+    int64_t PL, int KC, int mode, double scale, int64_t P1, void *stream)
 {
 	const int64_t total = R * nkb * PL;
 	if (total <= 0)
 		return 0;
 	int64_t nb = (total + 255) / 256;
 	k_pack<<<(unsigned)(nb < 65536 ? nb : 65536), 256, 0,
-	    (hipStream_t)stream>>>(src, dst, R, K, nkb, PL, KC, mode, scale);
+	    (hipStream_t)stream>>>(src, dst, R, K, nkb, PL, KC, mode, scale, P1);
 	return (int)hipGetLastError();
 }
 

@@ -118,7 +118,7 @@ pt_terms_same(pt_term_list &out, int set, int64_t o, int64_t v)
 
 void
 pt_terms_mixed(pt_term_list &out, int set, int64_t oa, int64_t va,
-    int64_t ob, int64_t vb)
+    int64_t ob, int64_t vb, bool vc_t)
 {
 	const int b0 = 12 * set;
 	const int TA = b0 + 0, TB = b0 + 1, TC = b0 + 2;
@@ -128,13 +128,15 @@ pt_terms_mixed(pt_term_list &out, int set, int64_t oa, int64_t va,
 	const int64_t PA = va * (va - 1) / 2;
 	const int ka = (int)va, kb = (int)vb, koa = (int)oa, kob = (int)ob;
 
-	/* src/pt.c:361-367: -sum_d TA[i,j,a,d] VC[k,d,c,b] + (a<->b) */
+	/* src/pt.c:361-367: -sum_d TA[i,j,a,d] VC[k,d,c,b] + (a<->b); the
+	 * plane (c, b) of VC, b contiguous, or transposed (b, c) */
+	const int64_t vcu = vc_t ? vb : 1, vcw = vc_t ? 1 : va;
 	out.push_back(TermBuilder(A, set, -1, ka)
 	    .S(TA, I, oa * va * va, J, va * va, va)
-	    .Q(VC, K, va * vb * va, vb * va).st(B, 1).st(C, va).done());
+	    .Q(VC, K, va * vb * va, vb * va).st(B, vcu).st(C, vcw).done());
 	out.push_back(TermBuilder(B, set, +1, ka)
 	    .S(TA, I, oa * va * va, J, va * va, va)
-	    .Q(VC, K, va * vb * va, vb * va).st(A, 1).st(C, va).done());
+	    .Q(VC, K, va * vb * va, vb * va).st(A, vcu).st(C, vcw).done());
 	/* :369-375: -sum_d TB[i,k,a,d] VB[j,d,b,c] + (a<->b) */
 	out.push_back(TermBuilder(A, set, -1, kb)
 	    .S(TB, I, ob * va * vb, K, va * vb, vb)

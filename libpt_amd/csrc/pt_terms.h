@@ -38,9 +38,11 @@ typedef FixedVec<pt_term, PT_MAX_TERMS> pt_term_list;
 
 /* same-spin case of sizes (o, v): 18 terms for W set `set` */
 void pt_terms_same(pt_term_list &out, int set, int64_t o, int64_t v);
-/* mixed-spin case, pair spin (oa, va), odd spin (ob, vb): 16 terms */
+/* mixed-spin case, pair spin (oa, va), odd spin (ob, vb): 16 terms.
+ * vc_t: the VC operand is packed with its plane transposed ([b][c], c
+ * contiguous), so that the pairs of one DMA row are contiguous */
 void pt_terms_mixed(pt_term_list &out, int set, int64_t oa,
-    int64_t va, int64_t ob, int64_t vb);
+    int64_t va, int64_t ob, int64_t vb, bool vc_t);
 /* order by W set, then role a, role b (phase 1), role c (phase 2), stable
  * within each group; returns the number of phase-1 terms of one set,
  * *n_role_a the number of role-a terms of one set */

@@ -173,3 +173,32 @@ def test_batch_graph_replays():
     finally:
         _close(systems)
         _close(other)
+
+
+@pytest.mark.parametrize("form", ["fp64", "bf16x3"])
+def test_batch_mixed_geometry(form):
+    """Systems recorded with different tile geometries (4 and 8 waves) in
+    one (fT) batch: the batch's save areas fit the larger geometry, and
+    every system's energy equals its plan's alone (which reads the same
+    records) and matches the oracle."""
+    dims_list = [(5, 31), (4, 19), (3, 41)]
+    geo = [4, 8, 4] if form == "fp64" else [8, 4, 8]
+    for d, w, blk in zip(dims_list, geo, [(2, 2, 8), (8, 8, 1), (4, 4, 2)]):
+        api.tune_record("rft", form, d, blk, w)
+    systems = _systems("rft", dims_list, seed0=41, sig_f=1e-2, omega=0.3)
+    try:
+        with api.Batch("rft", form, [(d, dev) for d, _, dev in systems]) as b:
+            b.run()
+            eb = b.energies()
+        for (d, spec, dev), e, w in zip(systems, eb, geo):
+            with api.Plan("rft", form, d, dev) as pl:
+                assert pl.geometry()[0] == w
+                pl.run_all()
+                ea = pl.energy()
+            eo = oracle.energy("rft", d, api.synth_host("rft", spec))
+            print(form, d, "waves", w, "batch", e, "alone", ea, "oracle", eo)
+            assert e == ea
+            tol = 1e-12 if form == "fp64" else 5e-4
+            assert abs(e - eo) <= max(1e-10, tol * abs(eo))
+    finally:
+        _close(systems)

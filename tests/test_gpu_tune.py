@@ -1,7 +1,11 @@
-"""Tile-order blocking and its autotuner (libpt_amd_plan_set_block /
+"""Tile geometry, tile-order blocking and their autotuner
+(libpt_amd_plan_set_geometry / libpt_amd_plan_set_block /
 libpt_amd_plan_tune): the order of the tiles changes only the summation
 order of the per-tile energies, so every blocking gives the same energy to
-rounding, and the tuned plan matches the oracle."""
+rounding; the two geometries (4 waves with 16-wide c tiles, 8 waves with
+32-wide c tiles) tile (a, b, c) differently and both match the oracle; the
+tuned plan matches the oracle, and a new plan of the same sizes takes the
+tuned geometry and blocking from the cache."""
 
 import numpy as np
 import pytest
@@ -43,11 +47,14 @@ def test_tune_picks_candidate_and_keeps_parity():
         with api.Plan(kind, "fp64", dims, dev) as pl:
             blk = pl.tune(sample_ijk=4)
             assert blk in CANDIDATES
+            waves = pl.geometry()[0]
+            assert waves in (4, 8)
             pl.run_all()
             e = pl.energy()
-        # a new plan of the same sizes takes the tuned blocking from the
-        # cache and gives the same energy bit for bit
+        # a new plan of the same sizes takes the tuned geometry and
+        # blocking from the cache and gives the same energy bit for bit
         with api.Plan(kind, "fp64", dims, dev) as pl2:
+            assert pl2.geometry()[0] == waves
             pl2.run_all()
             assert pl2.energy() == e
     finally:
@@ -55,3 +62,37 @@ def test_tune_picks_candidate_and_keeps_parity():
     eo = oracle.energy(kind, dims, api.synth_host(kind, spec))
     print("tuned", blk, e, "oracle", eo)
     assert abs(e - eo) <= max(1e-10, 1e-13 * abs(eo))
+
+
+@pytest.mark.parametrize("kind,dims,form,kw,tol", [
+    ("rpt", (6, 40), "fp64", dict(sig_f=1e-2), 1e-12),
+    ("upt", (5, 36, 4, 41), "bf16x3", dict(sig_f=1e-2), 1e-4),
+    ("rft", (6, 30), "fp64", dict(sig_f=1e-2, omega=0.3), 1e-12),
+    ("uft", (4, 18, 3, 21), "f16x3", dict(sig_f=1e-2, omega=0.3), 1e-4),
+    ("rpt", (6, 40), "fp32", dict(sig_f=1e-2), 1e-5),
+])
+def test_geometry_switch_in_place(kind, dims, form, kw, tol):
+    """One plan run in one geometry, switched to the other (tile lists,
+    item slots and (fT) save areas rebuilt) and back: every run matches
+    the oracle, and switching back gives the first energy bit for bit."""
+    spec = api.make_spec(kind, dims, seed=23, **kw)
+    arr = api.synth_host(kind, spec)
+    eo = oracle.energy(kind, dims, arr, mp=form == "fp32")
+    dev = api.to_device(kind, arr, fp32=form == "fp32")
+    try:
+        with api.Plan(kind, form, dims, dev) as pl:
+            w0 = pl.geometry()[0]
+            es = []
+            for w in (w0, 12 - w0, w0):
+                pl.set_geometry(w)
+                assert pl.geometry()[0] == w
+                pl.run_all()
+                es.append(pl.energy())
+            with pytest.raises(api.LibptError):
+                pl.set_geometry(6)
+    finally:
+        dev.close()
+    print(kind, form, "waves", w0, 12 - w0, es, "oracle", eo)
+    assert es[2] == es[0]
+    for e in es:
+        assert abs(e - eo) <= tol * abs(eo)
