@@ -268,26 +268,34 @@ def omp_set_threads(n):
     ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
 
 
-def cpu_baseline(api, o, v, seed, full, check=False):
+def cpu_baseline(api, o, vs, seed, full, check=False):
     """The reference libpt compiled from its sources (OpenMP threads spread
-    over the cores, MKL in sequential mode) on a bounded sample: the whole
-    o=45 ijk list at a reduced v, extrapolated to the full v by the
-    reference formulation's flop ratio; falls back to the oracle.  Also
-    the reference's own thread spread: one small call (rpt o=8 v=64) on
-    all threads and on one thread."""
+    over the cores, MKL in sequential mode) on a bounded sample: whole calls
+    over the full o=45 ijk list at reduced v (each v of `vs` in turn, one
+    progress line each), the rate of the last (largest) one extrapolated to
+    the full v by the reference formulation's flop ratio; falls back to the
+    oracle.  Also the reference's own thread spread: one small call (rpt
+    o=8 v=64) on all threads and on one thread."""
     os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
     os.environ.setdefault("OMP_PROC_BIND", "spread")
     os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
     threads = ref_threads()
-    spec = api.make_spec("rpt", (o, v), seed=seed)
-    arr = api.synth_host("rpt", spec)
     kind = "reference" if oracle.ref_available() else "port"
     fn = oracle.ref_energy if kind == "reference" else oracle.energy
-    t0 = time.perf_counter()
-    e = fn("rpt", (o, v), arr)
-    dt = time.perf_counter() - t0
-    f = api.flop("rpt", (o, v))
+    runs = []
+    for v in vs:
+        arr = api.synth_host("rpt", api.make_spec("rpt", (o, v), seed=seed))
+        t0 = time.perf_counter()
+        e = fn("rpt", (o, v), arr)
+        dt = time.perf_counter() - t0
+        f = api.flop("rpt", (o, v))
+        runs.append({"v": v, "flop": f, "s": round(dt, 2),
+                     "tflops": round(f / dt / 1e12, 4), "E": e})
+        log(f"[bench] cpu baseline: libpt_rpt o={o} v={v} on {threads} "
+            f"threads, {dt:.1f} s, {f / dt / 1e12:.3f} TFLOP/s")
+    v, f, dt, e = runs[-1]["v"], runs[-1]["flop"], runs[-1]["s"], \
+        runs[-1]["E"]
     f_full = api.flop("rpt", full)
     rate = f / dt
     out = {"value": round(rate / 1e12, 4), "unit": "TFLOP/s",
@@ -299,6 +307,7 @@ def cpu_baseline(api, o, v, seed, full, check=False):
            "sample": f"libpt_rpt o={o} v={v} FP64 (full call: every ijk "
                      f"triple of o={o} at v={v}, {f:.3e} flop, {dt:.1f} s, "
                      f"E={e:.6e})",
+           "samples": runs,
            "extrapolated_full_s": {
                "dims": list(full), "s": round(f_full / rate, 1),
                "how": "flop ratio of the reference formulation "
@@ -313,10 +322,13 @@ def cpu_baseline(api, o, v, seed, full, check=False):
         out["thread_spread"] = {
             "call": "libpt_rpt o=8 v=64", f"E_{threads}_threads": e_n,
             "E_1_thread": e_1, "abs_dE": abs(e_n - e_1)}
+        log(f"[bench] cpu baseline: thread spread at rpt o=8 v=64, "
+            f"|dE| {abs(e_n - e_1):.2e} Eh")
     if check:
         # the same sample through the GPU drop-in (host arrays in), checked
         # against the CPU energy; opt-in, since its launches share the tile
         # kernels and would mix into a profiled default run's statistics
+        arr = api.synth_host("rpt", api.make_spec("rpt", (o, v), seed=seed))
         t0 = time.perf_counter()
         e_gpu = api.rpt(o, v, *arr)
         gdt = time.perf_counter() - t0
@@ -509,9 +521,10 @@ def main():
                     "against their plans run one after another; off by "
                     "default, since its launches share the tile kernels and "
                     "would mix into a profiled default run's statistics")
-    ap.add_argument("--cpu", default="45,64",
-                    help="o,v of the CPU-baseline sample (every ijk triple "
-                    "of o at a reduced v, extrapolated to --o/--v by the "
+    ap.add_argument("--cpu", default="45,32,48",
+                    help="o,v1[,v2...] of the CPU-baseline sample (whole "
+                    "calls over every ijk triple of o at each reduced v; "
+                    "the last one's rate is extrapolated to --o/--v by the "
                     "flop ratio); 'none' to skip")
     ap.add_argument("--fixtures", type=int, default=1,
                     help="run the split forms of --forms on libpt's five "
@@ -667,8 +680,8 @@ def main():
                                            args.tune, args.tune_sample)
             log(f"[bench] batch {form}: {out['batch'][form]}")
     if args.cpu != "none":
-        co, cv = (int(x) for x in args.cpu.split(","))
-        out["cpu_baseline"] = cpu_baseline(api, co, cv, args.seed,
+        cs = [int(x) for x in args.cpu.split(",")]
+        out["cpu_baseline"] = cpu_baseline(api, cs[0], cs[1:], args.seed,
                                            (args.o, args.v), args.cpu_check)
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)

@@ -86,10 +86,11 @@ double libpt_amd_plan_flop_per_ijk(const libpt_amd_plan *plan, int sc);
 double libpt_amd_plan_factor(const libpt_amd_plan *plan, int sc);
 /* tile-kernel geometry of the plan: out3 = (waves per workgroup, c tile
  * width, ijk per item group).  Default: 4 waves (32 x 32 x 16 tiles for
- * FP64 and the split forms, 32 x 32 x 32 for FP32), unless the tuning
- * cache holds a geometry for the plan's sizes; no grouping of consecutive
- * ijk (1).  LIBPT_AMD_WAVES=4|8 and LIBPT_AMD_KGROUP=n override them for
- * new plans. */
+ * FP64, 32 x 32 x 32 for FP32 and the split forms), unless the tuning
+ * cache holds a geometry for the plan's sizes; items taken in groups of 4
+ * consecutive ijk (tile-major within a group, so that the items of one
+ * tile that share the slabs of i and j run together).  LIBPT_AMD_WAVES=4|8
+ * and LIBPT_AMD_KGROUP=n (1 to 64) override them for new plans. */
 int libpt_amd_plan_geometry(const libpt_amd_plan *plan, int *out3);
 
 /* enqueue ijk items [ijk_begin, ijk_end) of spin case sc on the plan's
@@ -108,9 +109,9 @@ int libpt_amd_plan_ijk_list(const libpt_amd_plan *plan, int sc, int32_t *out);
 /* Tile geometry and order of the work list.  Tiles (A, B, C) are visited
  * in blocks of ga x gb x gc so that workgroups running together share
  * operand slices in L2; set_block applies one blocking.  set_geometry
- * switches the tile kernel between 4 waves (32 x 32 x 16 tiles in FP64
- * and the split forms, 32 x 32 x 32 in FP32) and 8 waves (32 x 32 x 32
- * tiles), rebuilding the plan's tile lists (not for the plans of a
+ * switches the tile kernel between 4 waves (two workgroups per CU;
+ * 32 x 32 x 16 tiles in FP64, 32 x 32 x 32 in the other forms) and 8
+ * waves (one workgroup per CU; 32 x 32 x 32 tiles), rebuilding the plan's tile lists (not for the plans of a
  * batch).  tune times both geometries (only the plan's own when
  * LIBPT_AMD_WAVES is set) and the candidate blockings on the first
  * `sample_ijk` ijk triples of each spin case (0: 8), applies the fastest

@@ -61,8 +61,10 @@ pt_kgroups(int form, int64_t K)
 }
 
 /* Workgroup geometry (the kernel's Geo): NW = 4 waves, tile 32 a x 32 b x
- * TC c with TC = 32 for FP32 and 16 for FP64 and the split forms, two
- * workgroups per CU; or NW = 8 waves, tile 32 a x 32 b x 32 c, one
+ * TC c with TC = 16 for FP64 (whose 16 x 16 x 16 accumulator block per
+ * wave fills 128 registers) and 32 for FP32 and the split forms (FP32
+ * accumulators), two workgroups per CU; or NW = 8 waves, tile 32 a x 32 b
+ * x 32 c, one
  * workgroup per CU with four staging slots.  Each wave owns 16 a x 16 b x
  * CW c (CW = TC for four waves, 16 for eight).  (fT) forms its two W sets
  * one after the other in the same registers, so the geometry does not
@@ -84,7 +86,7 @@ static inline int
 pt_tcw(int form, int nset, int nw)
 {
 	(void)nset;
-	return nw == 8 ? 32 : (form == 1 ? 32 : 16);
+	return nw == 8 || form != 0 ? 32 : 16;
 }
 
 static inline int

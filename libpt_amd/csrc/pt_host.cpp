@@ -302,7 +302,10 @@ struct Arena {
 struct libpt_amd_plan {
 	int call = 0, form = 0;
 	int nw = 4;                 /* waves per workgroup of the tile kernel */
-	int kgrp = 1;               /* ijk per item group of a launch */
+	int kgrp = 4;               /* ijk per item group of a launch: 4
+	                               measured 10 % ahead of 1 for FP64 and
+	                               4 % for split-BF16 at o=45, v=477 (2, 8
+	                               and 16 within 1 % of 4) */
 	bool vct = true;            /* VC packed a second time, transposed */
 	bool fixed_geometry = false; /* a batch's plan: its launches are
 	                                recorded, so the tuner keeps nw */
@@ -1315,8 +1318,9 @@ libpt_amd_tune_record(int call, int form, size_t oa, size_t va, size_t ob,
 
 /*
  * Autotuner of the tile geometry and order: for each tile geometry (4
- * waves with 16-wide c tiles in FP64 and the split forms, 8 waves with
- * 32-wide c tiles; only the plan's own for a plan in a batch) and each
+ * waves, two workgroups per CU, with 16-wide c tiles in FP64 and 32-wide
+ * in the other forms; 8 waves, one workgroup per CU, with 32-wide c
+ * tiles; only the plan's own for a plan in a batch) and each
  * candidate blocking, time the first `sample` ijk triples of every spin
  * case (after one untimed pass per geometry), weight each case by its ijk
  * count, keep the fastest, and record it in the tuning cache.  The

@@ -724,13 +724,16 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
 /* -------------------------------------------------------------- kernel */
 
 /*
- * Workgroup geometry, per wave count NW (pt_tc, pt_cw in pt_engine.h give
+ * Workgroup geometry, per wave count NW (pt_tcw, pt_cw in pt_engine.h give
  * the same numbers to the host):
- *   NW = 4: tile 32 a x 32 b x TC c, TC = 32 for FP32 (whose 32 accumulator
- *           blocks of 4 registers fit beside a step's operands), 16 for FP64
- *           and the split forms; each wave owns 16 a x 16 b x TC c; two
- *           slots, two workgroups per CU, so one's staging and barriers
- *           overlap the other's MFMAs.
+ *   NW = 4: tile 32 a x 32 b x TC c, TC = 32 for FP32 and the split forms
+ *           (whose 32 FP32 accumulator blocks of 4 registers fit beside a
+ *           step's operands), 16 for FP64 (16 blocks of 8 registers); each
+ *           wave owns 16 a x 16 b x TC c; two slots, two workgroups per CU,
+ *           so one's staging and barriers overlap the other's MFMAs.  With
+ *           TC = 32 every step is type B and the role-c operand is reused
+ *           by 32 c (split-BF16 at o=45, v=477: 130 against 111 TFLOP/s
+ *           with TC = 16).
  *   NW = 8: tile 32 a x 32 b x 32 c, each wave 16 a x 16 b x 16 c (wave =
  *           wa + 2 wb + 4 wc); one workgroup per CU with four slots (three
  *           steps in flight).  Every step is type B (32 w per u-row); the
@@ -739,7 +742,7 @@ mma_p2(typename Form<F>::ACC (&ac)[TCW], const char *sS, const char *sQ,
  */
 template <int F, int NSET, int NW>
 struct Geo {
-	static constexpr int TC = NW == 8 ? 32 : (F == PT_FORM_FP32 ? 32 : 16);
+	static constexpr int TC = NW == 8 || F != PT_FORM_FP64 ? 32 : 16;
 	static constexpr int CW = NW == 8 ? 16 : TC;   /* c per wave */
 	static constexpr int THREADS = 64 * NW;
 	static constexpr int NS = NW == 8 ? 4 : 2;     /* staging slots */

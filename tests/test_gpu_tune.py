@@ -2,8 +2,9 @@
 (libpt_amd_plan_set_geometry / libpt_amd_plan_set_block /
 libpt_amd_plan_tune): the order of the tiles changes only the summation
 order of the per-tile energies, so every blocking gives the same energy to
-rounding; the two geometries (4 waves with 16-wide c tiles, 8 waves with
-32-wide c tiles) tile (a, b, c) differently and both match the oracle; the
+rounding; the two geometries (4 waves, 16-wide c tiles in FP64 and
+32-wide in the other forms; 8 waves, 32-wide c tiles) tile (a, b, c)
+differently and both match the oracle; the
 tuned plan matches the oracle, and a new plan of the same sizes takes the
 tuned geometry and blocking from the cache."""
 
