@@ -228,13 +228,17 @@ def run_config(api, name, kind, dims, frac, kw, forms, seed):
     the part run and its difference from the first form."""
     spec = api.make_spec(kind, dims, seed=seed, **kw)
     dev = api.synth_device(kind, spec)
+    # the FP32 (_mp) form reads float arrays of the same generator
+    dev32 = api.synth_device(kind, spec, fp32=True) if "fp32" in forms \
+        else None
     out = {"config": name, "call": kind, "dims": list(dims),
            "fraction_of_ijk": f"1/{frac}"}
     try:
         e0 = None
         for form in forms:
             t0 = time.perf_counter()
-            with api.Plan(kind, form, dims, dev) as pl:
+            with api.Plan(kind, form, dims,
+                          dev32 if form == "fp32" else dev) as pl:
                 prep = time.perf_counter() - t0
                 ms, flop, e = 0.0, 0.0, 0.0
                 for sc in range(pl.ncases):
@@ -258,6 +262,8 @@ def run_config(api, name, kind, dims, frac, kw, forms, seed):
             log(f"[bench] {name} {form}: {r}")
     finally:
         dev.close()
+        if dev32 is not None:
+            dev32.close()
     return out
 
 
