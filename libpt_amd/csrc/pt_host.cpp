@@ -739,6 +739,10 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 		pl->kgrp = std::max(1, std::min(64, atoi(g)));
 	if (const char *t = getenv("LIBPT_AMD_VC_TRANSPOSED"))
 		pl->vct = atoi(t) != 0;
+	/* the static scratch of the allocation-free call holds more systems
+	 * without the second copy of the mixed ovvv block */
+	if (pl->arena)
+		pl->vct = false;
 	pl->call = call;
 	pl->form = form;
 	pl->oa = oa;

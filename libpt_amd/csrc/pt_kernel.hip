@@ -326,7 +326,8 @@ bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
  * Staging: NSLOT slots, each holding one step (term, kappa chunk) of the
  * tile, loaded NSLOT - 1 steps ahead of the step computed.
  * Slot (bytes):
- *   S: 32 rows of 64 B of data (the chunk's kappa of one row) + padding;
+ *   S: 32 rows of the chunk's kappa, packed at their data width (64 B for
+ *      type A, 32 B for type B);
  *   Q: 32 u-rows of 1 KB of data + padding.  A u-row is the chunk of the
  *      tile's w range:
  *      type A (w range 16, roles a and b of a 16-wide c tile): 16 w x 64 B,
@@ -338,17 +339,20 @@ bload_ae(__amdgpu_buffer_rsrc_t r, uint32_t vofs, int sofs)
  * MFMA lane group lg reads chunk lg of a type-A row (16 B: kappa 2lg, 2lg+1
  * for FP64, 4lg .. 4lg+3 for FP32) or element lg of a type-B row (8 B: kappa
  * lg for FP64, 2lg, 2lg+1 for FP32), the same mapping for both operands;
- * lane n of a lane group reads row n.  The row strides are those at which
- * such reads have no LDS bank conflicts on gfx950 (measured with
+ * lane n of a lane group reads row n.  The Q row strides are those at
+ * which such reads have no LDS bank conflicts on gfx950 (measured with
  * SQ_LDS_BANK_CONFLICT): 16-byte reads over 16 rows need a stride of 32
- * modulo 64 bytes (type A: S 96, Q 1056), 8-byte reads 16 modulo 64 (type
- * B: S 80, Q 1040); the other strides cost a 2-way conflict.
+ * modulo 64 bytes (type A: 1056), 8-byte reads 16 modulo 64 (type B: 1040);
+ * the other strides cost a 2-way conflict.  S rows are read by one
+ * instruction per step, so their conflicts cost little, and packing them
+ * at their data width takes one DMA instruction per wave and step fewer
+ * (FP64 +1.4 % at o=45, v=477 against rows of 96 and 80 bytes).
  */
 template <int NW, int NS>
 struct Stg {
-	static constexpr int S_ROW_A = 96, S_ROW_B = 80;
+	static constexpr int S_ROW_A = 64, S_ROW_B = 32;
 	static constexpr int Q_USTR_A = 1024 + 32, Q_USTR_B = 1024 + 16;
-	static constexpr int S_BYTES = NW == 8 ? 4096 : 3072;
+	static constexpr int S_BYTES = 2048;
 	static constexpr int Q_BYTES = 32 * Q_USTR_A;
 	static constexpr int SLOT = S_BYTES + Q_BYTES;
 	static constexpr int NSLOT = NS;
@@ -407,19 +411,18 @@ wait_vm()
 
 /*
  * S rows by DMA as well: the S region of a slot holds 32 rows at stride
- * S_ROW_A (96) or S_ROW_B (80) bytes, the step's chunk of kappa (64 or 32
- * bytes, in the API element type) at the start of each row.  The region is
- * covered by NW x SD dword DMA instructions per step: dword p = 64 (SD wave
- * + d) + lane of the region is row p / (stride / 4), dword p % (stride / 4)
- * of it; padding dwords and rows outside the tile fetch an offset every
- * descriptor rejects (zero).  The term's sign is applied to the
+ * S_ROW_A (64) or S_ROW_B (32) bytes, the step's chunk of kappa (in the API
+ * element type).  The region is covered by NW x SD dword DMA instructions
+ * per step: dword p = 64 (SD wave + d) + lane of the region is row
+ * p / (stride / 4), dword p % (stride / 4) of it; rows outside the tile
+ * fetch an offset every descriptor rejects (zero).  The term's sign is applied to the
  * accumulators (an exact negation before and after its steps), not to S.
  * Kappa beyond the term's extent read the next S elements, which meet
  * the zero padding of the packed Q operand.
  */
 template <int NW>
 struct SDma {
-	static constexpr int SD = NW == 8 ? 2 : 3;   /* instructions per wave */
+	static constexpr int SD = NW == 8 ? 1 : 2;   /* instructions per wave */
 	static constexpr int BYTES = NW * SD * 256;  /* S region of a slot */
 };
 
@@ -513,12 +516,12 @@ rd_b16(const char *pf, const char *pv, v2u &f, v4u (&q)[8])
 	    : [pf] "v"(PT_A(pf)), [pv] "v"(PT_A(pv)) : "memory");
 }
 
-/* two S values (pf, pf + 1280: rows 16 apart) and 16 Q values */
+/* two S values (pf, pf + 512: rows 16 apart) and 16 Q values */
 __device__ __forceinline__ void
 rd_b16s2(const char *pf, const char *pv, v2u &f0, v2u &f1, v4u (&q)[8])
 {
 	asm volatile("ds_read_b64 %[f0], %[pf]\n\t"
-	    "ds_read_b64 %[f1], %[pf] offset:1280\n\t"
+	    "ds_read_b64 %[f1], %[pf] offset:512\n\t"
 	    PT_R2(0, 0, 4) PT_R2(1, 8, 12) PT_R2(2, 16, 20) PT_R2(3, 24, 28)
 	    PT_R2(4, 32, 36) PT_R2(5, 40, 44) PT_R2(6, 48, 52) PT_R2(7, 56, 60)
 	    "s_waitcnt lgkmcnt(0)"
@@ -779,6 +782,8 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	static_assert(SG::SLOT >= 4 * 4 * 16 * 17 * (int)sizeof(AE),
 	    "transpose round exceeds a slot");
 	static_assert(NSET == 1 || NSET == 2, "one or two W sets");
+	static_assert(SDma<NW>::BYTES <= SG::S_BYTES,
+	    "the S DMAs of a step stay inside the slot's S region");
 
 	__shared__ __attribute__((aligned(16))) char smem[SMEM];
 	__shared__ double red[NW];
