@@ -309,6 +309,8 @@ struct libpt_amd_plan {
 	bool vct = true;            /* VC packed a second time, transposed */
 	bool fixed_geometry = false; /* a batch's plan: its launches are
 	                                recorded, so the tuner keeps nw */
+	bool unusable = false;      /* a geometry or order change failed half
+	                               way: the tile lists are not whole */
 	int64_t oa = 0, va = 0, ob = 0, vb = 0;
 	hipStream_t stream = nullptr;
 	bool own_stream = true;     /* created (and destroyed) by the plan */
@@ -688,6 +690,8 @@ plan_set_geometry(libpt_amd_plan *pl, int nw)
 	}
 	HCHK(hipStreamSynchronize(pl->stream), false);
 	const bool had_wsave = pl->wsave != nullptr;
+	/* from here a failure leaves the plan without tile lists */
+	pl->unusable = true;
 	plan_drop(pl, pl->wsave);
 	pl->wsave = nullptr;
 	pl->wslots = 0;
@@ -711,6 +715,7 @@ plan_set_geometry(libpt_amd_plan *pl, int nw)
 	if (had_wsave && !plan_wsave(pl))
 		return false;
 	HCHK(hipDeviceSynchronize(), false);
+	pl->unusable = false;
 	return true;
 }
 
@@ -1149,6 +1154,10 @@ libpt_amd_plan_run(libpt_amd_plan *pl, int sc, int64_t b, int64_t e)
 		set_err("bad plan or spin case");
 		return -1;
 	}
+	if (pl->unusable) {
+		set_err("the plan lost its tile lists in a failed geometry or order change");
+		return -1;
+	}
 	Case &c = pl->cases[sc];
 	if (c.n_ijk == 0)
 		return 0;
@@ -1283,8 +1292,11 @@ libpt_amd_plan_set_block(libpt_amd_plan *pl, int ga, int gb, int gc)
 	tb.gc = gc;
 	HCHK(hipStreamSynchronize(pl->stream), -1);
 	for (Case &c : pl->cases)
-		if (!case_set_block(pl, c, tb))
+		if (!case_set_block(pl, c, tb)) {
+			/* a tile list may be half rewritten */
+			pl->unusable = true;
 			return -1;
+		}
 	return 0;
 }
 
@@ -1676,6 +1688,12 @@ libpt_amd_batch_run(libpt_amd_batch *b)
 		set_err("null batch");
 		return -1;
 	}
+	for (const libpt_amd_plan *pl : b->plans)
+		if (pl->unusable) {
+			set_err("a system's tile list was left half rewritten by a "
+			    "failed tuning step");
+			return -1;
+		}
 	HCHK(hipEventRecord(b->ev0, b->stream), -1);
 	HCHK(hipGraphLaunch(b->exec, b->stream), -1);
 	HCHK(hipEventRecord(b->ev1, b->stream), -1);
