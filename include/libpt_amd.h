@@ -90,7 +90,11 @@ double libpt_amd_plan_factor(const libpt_amd_plan *plan, int sc);
  * cache holds a geometry for the plan's sizes; items taken in groups of 4
  * consecutive ijk (tile-major within a group, so that the items of one
  * tile that share the slabs of i and j run together).  LIBPT_AMD_WAVES=4|8
- * and LIBPT_AMD_KGROUP=n (1 to 64) override them for new plans. */
+ * and LIBPT_AMD_KGROUP=n (1 to 64) override them for new plans;
+ * LIBPT_AMD_PERSISTENT=1 runs new plans' launches on a persistent grid
+ * (one workgroup per CU share, items from per-XCD counters, one (fT)
+ * save area per workgroup): the same energies bit for bit, slower than
+ * the default one workgroup per item. */
 int libpt_amd_plan_geometry(const libpt_amd_plan *plan, int *out3);
 
 /* enqueue ijk items [ijk_begin, ijk_end) of spin case sc on the plan's

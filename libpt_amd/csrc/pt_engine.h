@@ -149,6 +149,13 @@ struct pt_launch {
 	                                n_tiles) taken in groups of kgrp
 	                                consecutive ijk, each group's items
 	                                tile-major; 1: items in order */
+	uint32_t *ctr;               /* non-null: a persistent grid of pgrid
+	                                workgroups takes the items from these
+	                                8 counters (one per XCD, zeroed by the
+	                                launch), with one save area per
+	                                workgroup (nwsave >= pgrid); null: one
+	                                workgroup per item */
+	int32_t pgrid;
 	struct pt_epi epi;
 };
 

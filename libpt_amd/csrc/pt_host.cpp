@@ -311,6 +311,8 @@ struct libpt_amd_plan {
 	                                recorded, so the tuner keeps nw */
 	bool unusable = false;      /* a geometry or order change failed half
 	                               way: the tile lists are not whole */
+	bool persist = false;       /* persistent grid (LIBPT_AMD_PERSISTENT) */
+	uint32_t *d_ctr = nullptr;  /* its 8 item counters */
 	int64_t oa = 0, va = 0, ob = 0, vb = 0;
 	hipStream_t stream = nullptr;
 	bool own_stream = true;     /* created (and destroyed) by the plan */
@@ -399,6 +401,19 @@ plan_alloc(libpt_amd_plan *pl, void **p, size_t bytes)
 		return false;
 	pl->owned.push_back(*p);
 	return true;
+}
+
+/* workgroups of a persistent grid: every CU's share for the geometry,
+ * rounded up to a multiple of the 8 XCDs */
+int
+persist_grid(int nw)
+{
+	int ncu = 0;
+	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+	    g_dev) != hipSuccess || ncu <= 0)
+		ncu = 256;
+	const int g = ncu * (nw == 8 ? 1 : 2);
+	return (g + 7) / 8 * 8;
 }
 
 /* frees one of the plan's hipMalloc buffers (no-op for null) */
@@ -579,6 +594,8 @@ case_finish(libpt_amd_plan *pl, Case &c, int form, int kind, int nset,
 	pt_launch &L = c.L;
 	L.nw = pl->nw;
 	L.kgrp = pl->kgrp;
+	L.ctr = pl->persist ? pl->d_ctr : nullptr;
+	L.pgrid = persist_grid(pl->nw);
 	L.kind = kind;
 	L.nset = nset;
 	L.oa = oa;
@@ -659,7 +676,10 @@ plan_wsave(libpt_amd_plan *pl)
 		most = std::max(most, d.n_ijk * d.n_tiles);
 	int64_t slots = std::min<int64_t>(pt_ft_chunk(pl->form, pl->nw), most);
 	const int64_t wb = pt_wsave_bytes(pl->form, 2, pl->nw);
-	if (pl->arena) {
+	if (pl->persist)
+		/* one area per workgroup of the grid, whatever the item count */
+		slots = persist_grid(pl->nw);
+	else if (pl->arena) {
 		/* what the scratch has left, in whole areas: launches then hold
 		 * fewer items */
 		const int64_t left = (int64_t)(pl->arena->cap - pl->arena->used) / wb;
@@ -706,6 +726,7 @@ plan_set_geometry(libpt_amd_plan *pl, int nw)
 		c.d_partial = nullptr;
 		L.nw = nw;
 		L.tcw = pt_tcw(pl->form, L.nset, nw);
+		L.pgrid = persist_grid(nw);
 		if (!case_tiles(pl, c, L.nc, L.tcw))
 			return false;
 		L.tiles = c.d_tiles;
@@ -748,6 +769,8 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 	 * without the second copy of the mixed ovvv block */
 	if (pl->arena)
 		pl->vct = false;
+	if (const char *t = getenv("LIBPT_AMD_PERSISTENT"))
+		pl->persist = atoi(t) != 0;
 	pl->call = call;
 	pl->form = form;
 	pl->oa = oa;
@@ -767,6 +790,7 @@ plan_init(libpt_amd_plan *pl, int call, int form, size_t oa_, size_t va_,
 		    (set_err("hipEventCreate failed"), false));
 	}
 	PFAIL(plan_alloc(pl, (void **)&pl->d_out, 4 * sizeof(double)));
+	PFAIL(plan_alloc(pl, (void **)&pl->d_ctr, 8 * sizeof(uint32_t)));
 
 	int64_t n[10];
 	const int narr = api_sizes(call, oa, va, ob, vb, n);
@@ -1061,7 +1085,7 @@ launch_items(int form, const pt_launch &base, int64_t item_b, int64_t item_e,
     void *wsave, int64_t wslots, hipStream_t st)
 {
 	const int64_t nt = base.n_tiles;
-	int64_t chunk = base.nset == 2 ? wslots : (int64_t)1 << 22;
+	int64_t chunk = base.nset == 2 && !base.ctr ? wslots : (int64_t)1 << 22;
 	int kg = base.kgrp;
 	if (kg > 1) {
 		if (item_b % nt || item_e % nt || chunk < nt)
@@ -1562,6 +1586,11 @@ libpt_amd_batch_create(int call, int form, int n, const int64_t *dims,
 		for (const Seg &g : segs)
 			most = std::max(most, g.items);
 		b->wslots = std::min<int64_t>(((int64_t)4 << 30) / wbytes, most);
+		/* a persistent grid keeps one area per workgroup */
+		for (const libpt_amd_plan *pl : b->plans)
+			if (pl->persist)
+				b->wslots = std::max<int64_t>(b->wslots,
+				    persist_grid(pl->nw));
 		BFAIL(dev_alloc(&b->wsave, (size_t)b->wslots * (size_t)wbytes));
 	}
 	if (nseg > 0) {

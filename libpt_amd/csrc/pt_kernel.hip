@@ -1131,6 +1131,20 @@ wsave_of(void *wsave, int64_t slot)
  * item, handed out by the dispatcher in order; with two W sets item n of
  * the launch keeps W of set 0 in save area n (n_chunk <= p.nwsave).
  */
+/* work-list item of launch position it (kgrp > 1: groups of kgrp ijk,
+ * tile-major within a group) */
+__device__ __forceinline__ int64_t
+item_of(const pt_launch &p, int64_t it)
+{
+	if (p.kgrp <= 1)
+		return p.item0 + it;
+	const int64_t nt = p.n_tiles, G = p.kgrp, m = p.n_chunk / nt;
+	const int64_t grp = it / (G * nt), r = it - grp * G * nt;
+	const int64_t gsz = G < m - grp * G ? G : m - grp * G;
+	const int64_t tile = r / gsz, g = r - tile * gsz;
+	return p.item0 + (grp * G + g) * nt + tile;
+}
+
 template <int F, int KIND, int NSET, int NW>
 __global__ __launch_bounds__(64 * NW, NW == 8 ? 1 : 2) void
 k_tiles(pt_launch p)
@@ -1140,16 +1154,50 @@ k_tiles(pt_launch p)
 	const int64_t it = x * per + (blockIdx.x >> 3);
 	if (it >= p.n_chunk)
 		return;
-	int64_t item = p.item0 + it;
-	if (p.kgrp > 1) {
-		const int64_t nt = p.n_tiles, G = p.kgrp, m = p.n_chunk / nt;
-		const int64_t grp = it / (G * nt), r = it - grp * G * nt;
-		const int64_t gsz = G < m - grp * G ? G : m - grp * G;
-		const int64_t tile = r / gsz, g = r - tile * gsz;
-		item = p.item0 + (grp * G + g) * nt + tile;
-	}
-	tile_item<F, KIND, NSET, NW>(p, item,
+	tile_item<F, KIND, NSET, NW>(p, item_of(p, it),
 	    NSET == 2 ? wsave_of<F, NW>(p.wsave, it) : nullptr);
+}
+
+/*
+ * The same items from a persistent grid: workgroup w serves XCD w % 8 and
+ * takes that XCD's run of launch positions, [x per, x per + per), one at a
+ * time from the XCD's counter, in order.  The counter is fetched by one
+ * thread between two barriers that every wave meets outside any divergent
+ * branch (the previous item's partial-energy store is behind the first),
+ * the value is read back as a wave-uniform scalar, and the loop ends for
+ * the whole workgroup when it passes the run's end, which the host bounds;
+ * no workgroup waits on another.  With two W sets each workgroup keeps W of
+ * set 0 in its own save area (slot w).
+ */
+template <int F, int KIND, int NSET, int NW>
+__global__ __launch_bounds__(64 * NW, NW == 8 ? 1 : 2) void
+k_tiles_p(pt_launch p)
+{
+	__shared__ int s_next;
+	const int64_t per = (p.n_chunk + 7) / 8;
+	const int x = blockIdx.x & 7;
+	const int64_t lo = x * per;
+	const int64_t cnt = p.n_chunk - lo < per ? p.n_chunk - lo : per;
+	for (;;) {
+		__syncthreads();
+		if (threadIdx.x == 0)
+			s_next = (int)atomicAdd(p.ctr + x, 1u);
+		__syncthreads();
+		const int64_t r = __builtin_amdgcn_readfirstlane(s_next);
+		if (r >= cnt)
+			break;
+		tile_item<F, KIND, NSET, NW>(p, item_of(p, lo + r),
+		    NSET == 2 ? wsave_of<F, NW>(p.wsave, blockIdx.x) : nullptr);
+	}
+}
+
+/* the 8 item counters of a persistent launch, zeroed on its stream by a
+ * kernel (a graph may record it) */
+__global__ void
+k_zero_ctr(uint32_t *c)
+{
+	if (threadIdx.x < 8)
+		c[threadIdx.x] = 0u;
 }
 
 /* fixed-order compensated sum of x[0..n) by one workgroup of 1024 */
@@ -1327,11 +1375,26 @@ int
 launch_kind(const pt_launch *p, hipStream_t s)
 {
 	typedef Geo<F, NSET, NW> GM;
-	/* one save area per item; the host's tile width must be the kernel's */
-	if ((NSET == 2 && (!p->wsave || p->n_chunk > p->nwsave)) ||
-	    p->tcw != GM::TC)
+	if (p->tcw != GM::TC)
 		return (int)hipErrorInvalidValue;
-	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8))), block(GM::THREADS);
+	dim3 block(GM::THREADS);
+	if (p->ctr) {
+		/* persistent grid: one save area per workgroup */
+		if (p->pgrid <= 0 || p->pgrid % 8 || p->n_chunk > INT32_MAX / 2 ||
+		    (NSET == 2 && (!p->wsave || p->pgrid > p->nwsave)))
+			return (int)hipErrorInvalidValue;
+		k_zero_ctr<<<1, 64, 0, s>>>(p->ctr);
+		dim3 grid((unsigned)p->pgrid);
+		if (p->kind == PT_AAA)
+			k_tiles_p<F, PT_AAA, NSET, NW><<<grid, block, 0, s>>>(*p);
+		else
+			k_tiles_p<F, PT_AAB, NSET, NW><<<grid, block, 0, s>>>(*p);
+		return (int)hipGetLastError();
+	}
+	/* one save area per item */
+	if (NSET == 2 && (!p->wsave || p->n_chunk > p->nwsave))
+		return (int)hipErrorInvalidValue;
+	dim3 grid((unsigned)(8 * ((p->n_chunk + 7) / 8)));
 	if (p->kind == PT_AAA)
 		k_tiles<F, PT_AAA, NSET, NW><<<grid, block, 0, s>>>(*p);
 	else
