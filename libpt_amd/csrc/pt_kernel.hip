@@ -59,7 +59,7 @@ template <> struct Form<PT_FORM_FP64> {
 	typedef double AE;  /* accumulator element */
 	typedef v4d ACC;
 	typedef double ER;  /* epilogue real */
-	static constexpr int NP = 1, KC = 8, KS = 4;
+	static constexpr int KC = 8;    /* kappa per type-A step */
 	static constexpr bool ROW_F64 = true, SPLIT = false;
 };
 
@@ -69,7 +69,7 @@ template <> struct Form<PT_FORM_FP32> {
 	typedef float AE;
 	typedef v4f ACC;
 	typedef float ER;
-	static constexpr int NP = 1, KC = 16, KS = 4;
+	static constexpr int KC = 16;
 	static constexpr bool ROW_F64 = false, SPLIT = false;
 };
 
@@ -79,7 +79,7 @@ template <> struct Form<PT_FORM_BF16X3> {
 	typedef float AE;
 	typedef v4f ACC;
 	typedef double ER;
-	static constexpr int NP = 1, KC = 16, KS = 16;
+	static constexpr int KC = 16;
 	static constexpr bool ROW_F64 = false, SPLIT = true;
 };
 
@@ -220,9 +220,8 @@ struct Step {
 	int64_t sbytes;      /* S rows [0, sext) from sb0 */
 	int64_t qgs;         /* bytes per kappa group of the packed slab */
 	int64_t s_st, q_u, q_w;
-	int role, set, K, packed, ufast;
-	int s0, sn, sext, u0, uext, w0, wext, qwe, qsh;
-	int lo_mask, hi_sh;  /* pair index split: fast index bits, shift */
+	int role, set, K, packed;
+	int s0, sn, sext, u0, uext, w0, wext;
 };
 
 /* T: the workgroup's LDS copy of a term; every field is read as a
@@ -237,27 +236,23 @@ make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
 	s.set = uni(T.set);
 	s.K = uni(T.K);
 	s.packed = uni(T.packed);
-	s.ufast = uni(T.ufast);
 	s.s_st = uni64(T.s_st);
 	s.qgs = uni64(T.q_k) * GB;
 	const int na = p.na, nc = p.nc;
 	int ur, wr;
 	if (s.role == PT_ROLE_A) {
 		s.s0 = A0; s.sn = 32; s.sext = na;
-		s.u0 = B0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
+		s.u0 = B0; s.uext = na; s.w0 = C0; s.wext = nc;
 		ur = PT_ROLE_B; wr = PT_ROLE_C;
 	} else if (s.role == PT_ROLE_B) {
 		s.s0 = B0; s.sn = 32; s.sext = na;
-		s.u0 = A0; s.uext = na; s.w0 = C0; s.wext = nc; s.qwe = tcw;
+		s.u0 = A0; s.uext = na; s.w0 = C0; s.wext = nc;
 		ur = PT_ROLE_A; wr = PT_ROLE_C;
 	} else {
 		s.s0 = C0; s.sn = tcw; s.sext = nc;
-		s.u0 = A0; s.uext = na; s.w0 = B0; s.wext = na; s.qwe = 32;
+		s.u0 = A0; s.uext = na; s.w0 = B0; s.wext = na;
 		ur = PT_ROLE_A; wr = PT_ROLE_B;
 	}
-	s.qsh = s.qwe == 32 ? 5 : 4;
-	s.lo_mask = s.ufast ? 31 : s.qwe - 1;
-	s.hi_sh = s.ufast ? 5 : s.qsh;
 	s.q_u = uni64(T.q_st[ur]);
 	s.q_w = uni64(T.q_st[wr]);
 	const int64_t soff = uni64(T.s_off[0]) * i + uni64(T.s_off[1]) * j +
@@ -268,15 +263,6 @@ make_step(const pt_launch &p, const pt_term &T, int i, int j, int k, int A0,
 	s.qb0 = (const char *)uni64((int64_t)p.buf[uni(T.q_buf)]) + qoff;
 	s.sbytes = (int64_t)s.sext * s.s_st * ES;
 	return s;
-}
-
-/* (u, w) of pair index pidx: the contiguous index of Q varies fastest */
-__device__ __forceinline__ void
-pair_uw(const Step &s, int pidx, int &u, int &w)
-{
-	const int lo = pidx & s.lo_mask, hi = pidx >> s.hi_sh;
-	u = s.ufast ? lo : hi;
-	w = s.ufast ? hi : lo;
 }
 
 /* buffer descriptors: loads at offsets at or beyond the descriptor's byte
