@@ -131,6 +131,16 @@ def run_form(api, form, dims, dev, args, nbatch):
     dom_flop = sum((rng(dom, b)[1] - rng(dom, b)[0]) * fpi[dom]
                    for b in batches)
     waves, tcw, kgrp = plan.geometry()
+    # per spin case: launch time per step and its own rate
+    cases = []
+    for sc in range(ncase):
+        f = sum((rng(sc, b)[1] - rng(sc, b)[0]) * fpi[sc] for b in batches)
+        if kern_ms[sc] > 0:
+            cases.append({"case": "aaa" if sc == 0 else "aab",
+                          "ijk_per_step": rng(sc, batches[0])[1] -
+                          rng(sc, batches[0])[0],
+                          "launch_ms": round(kern_ms[sc] / args.steps, 2),
+                          "tflops": round(f / kern_ms[sc] / 1e9, 2)})
     res = dict(form=form, prep_s=prep, tile_block=blk, tune_s=tune_s,
                geometry={"waves": waves, "tile": f"32x32x{tcw}",
                          "ijk_group": kgrp},
@@ -139,7 +149,7 @@ def run_form(api, form, dims, dev, args, nbatch):
                kern_ms=kern_ms, dom_case=dom,
                dom_launch_ms=kern_ms[dom] / args.steps,
                dom_flop_per_launch=dom_flop / args.steps,
-               e_batches=e_batches,
+               e_batches=e_batches, cases=cases,
                total_flop=sum(n * f for n, f in zip(nijk, fpi)))
     plan.close()
     return res
@@ -643,6 +653,7 @@ def main():
             "projected_full_T_wall_s": round(proj_wall, 1),
             "tile_block": r["tile_block"],
             "geometry": r["geometry"],
+            "spin_cases": r["cases"],
         },
         "roofline": {
             "bound": "mfma",
@@ -662,6 +673,7 @@ def main():
             "speedup_vs_" + main_form: round(q["tflops"] / r["tflops"], 3),
             "tile_block": q["tile_block"],
             "geometry": q["geometry"],
+            "spin_cases": q["cases"],
             "projected_full_T_wall_s": round(q["total_flop"] /
                                              (q["tflops"] * 1e12), 1),
             "mfma_frac": round(q["dom_flop_per_launch"] * EXEC_FACTOR[form] /
