@@ -5,6 +5,9 @@
  *   - a choice of arithmetic form (FP64, FP32, split BF16 / FP16),
  *   - inputs that already live in GPU memory (no host copy),
  *   - running a subset of the ijk work list and reading per-ijk energies,
+ *   - the tile geometry and order of the kernel, and their autotuner,
+ *   - batches of many small systems in one recorded graph,
+ *   - a call that allocates no device memory,
  *   - the deterministic synthetic-input generator used by tests and bench.
  *
  * Array arguments are always passed in pt.h order:
