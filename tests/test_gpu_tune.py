@@ -8,6 +8,8 @@ differently and both match the oracle; the
 tuned plan matches the oracle, and a new plan of the same sizes takes the
 tuned geometry and blocking from the cache."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -97,3 +99,32 @@ def test_geometry_switch_in_place(kind, dims, form, kw, tol):
     assert es[2] == es[0]
     for e in es:
         assert abs(e - eo) <= tol * abs(eo)
+
+
+def test_plans_read_records_from_the_cache_file(tmp_path):
+    """A record in LIBPT_AMD_TUNE_CACHE (written by an earlier process)
+    sets the geometry and blocking of new plans of its sizes in a fresh
+    process; older 9-field lines (no geometry) keep the default geometry."""
+    import subprocess
+    import sys
+    import textwrap
+    from conftest import ROOT
+    cache = tmp_path / "tune.txt"
+    cache.write_text("0 0 6 45 6 45 4 4 4\n0 0 7 45 7 45 2 2 8 8\n")
+    code = textwrap.dedent("""
+        from libpt_amd import api
+        for dims in ((6, 45), (7, 45)):
+            spec = api.make_spec("rpt", dims, seed=3, sig_f=1e-2)
+            dev = api.synth_device("rpt", spec)
+            with api.Plan("rpt", "fp64", dims, dev) as pl:
+                print(dims, pl.geometry()[0], flush=True)
+            dev.close()
+    """)
+    env = dict(os.environ, LIBPT_AMD_TUNE_CACHE=str(cache))
+    env.pop("LIBPT_AMD_WAVES", None)
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "(6, 45) 4" in r.stdout
+    assert "(7, 45) 8" in r.stdout
