@@ -22,6 +22,7 @@
 #include <atomic>
 #include <mutex>
 #include <string>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -1996,8 +1997,14 @@ h2d_copy(void *const *dst, const void *const *src, const size_t *bytes,
 			(void)hipStreamDestroy(st);
 	};
 	std::vector<std::thread> th;
-	for (int w = 0; w < nw; ++w)
-		th.emplace_back(worker);
+	try {
+		for (int w = 0; w < nw; ++w)
+			th.emplace_back(worker);
+	} catch (const std::exception &) {
+		/* the workers started take every chunk; with none, copy here */
+		if (th.empty())
+			worker();
+	}
 	for (std::thread &t : th)
 		t.join();
 	if (failed) {
