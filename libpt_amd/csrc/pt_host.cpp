@@ -20,9 +20,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <mutex>
 #include <string>
-#include <exception>
 #include <thread>
 #include <vector>
 
