@@ -326,8 +326,9 @@ struct libpt_amd_plan {
 	double *d_out = nullptr;
 	double scale[4] = {1, 1, 1, 1};
 	double prep_s = 0;
-	/* two W sets: save areas of one launch's items, allocated at the
-	 * first run (wslots of pt_wsave_bytes each) */
+	/* two W sets: save areas of one launch's items (wslots of
+	 * pt_wsave_bytes each), allocated with the plan, or rebuilt by a
+	 * geometry change; a batch's plans point at the batch's */
 	void *wsave = nullptr;
 	int64_t wslots = 0;
 };
