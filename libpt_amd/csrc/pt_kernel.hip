@@ -142,48 +142,6 @@ mma_split(v4u a, v4u b, v4f c)
 
 /* ---------------------------------------------------------- epilogue */
 
-template <class ER>
-__device__ __forceinline__ ER
-z_same(const ER *u, const ER *g, int64_t o, int64_t v, int i, int j, int k,
-    int a, int b, int c)
-{
-	/* src/pt.c:162-175 */
-	const int64_t vv = v * v;
-	const ER *gjk = g + ((int64_t)j * o + k) * vv;
-	const ER *gik = g + ((int64_t)i * o + k) * vv;
-	const ER *gji = g + ((int64_t)j * o + i) * vv;
-	return +u[i * v + a] * gjk[b * v + c]
-	       -u[j * v + a] * gik[b * v + c]
-	       -u[k * v + a] * gji[b * v + c]
-	       -u[i * v + b] * gjk[a * v + c]
-	       +u[j * v + b] * gik[a * v + c]
-	       +u[k * v + b] * gji[a * v + c]
-	       -u[i * v + c] * gjk[b * v + a]
-	       +u[j * v + c] * gik[b * v + a]
-	       +u[k * v + c] * gji[b * v + a];
-}
-
-template <class ER>
-__device__ __forceinline__ ER
-z_mixed(const pt_epi &e, int64_t oa, int64_t va, int64_t ob, int64_t vb,
-    int i, int j, int k, int a, int b, int c)
-{
-	/* src/pt.c:177-185 and :433-442 */
-	const ER *u1 = (const ER *)e.u1, *u2 = (const ER *)e.u2;
-	const ER *f1 = (const ER *)e.f1, *f2 = (const ER *)e.f2;
-	const ER *GA = (const ER *)e.GA, *GB = (const ER *)e.GB;
-	const ER *ZA = (const ER *)e.ZA, *ZB = (const ER *)e.ZB;
-	const int64_t jk = ((int64_t)j * ob + k) * va, ik = ((int64_t)i * ob + k) * va;
-	const int64_t jkbc = (jk + b) * vb + c, jkac = (jk + a) * vb + c;
-	const int64_t ikbc = (ik + b) * vb + c, ikac = (ik + a) * vb + c;
-	const int64_t jiba = (((int64_t)j * oa + i) * va + b) * va + a;
-	return (u1[i * va + a] * GB[jkbc] + f1[i * va + a] * ZB[jkbc])
-	     - (u1[i * va + b] * GB[jkac] + f1[i * va + b] * ZB[jkac])
-	     - (u1[j * va + a] * GB[ikbc] + f1[j * va + a] * ZB[ikbc])
-	     + (u1[j * va + b] * GB[ikac] + f1[j * va + b] * ZB[ikac])
-	     + (u2[k * vb + c] * GA[jiba] + f2[k * vb + c] * ZA[jiba]);
-}
-
 __device__ __forceinline__ void
 kahan(double &s, double &cmp, double x)
 {
@@ -191,6 +149,120 @@ kahan(double &s, double &cmp, double x)
 	double t = s + y;
 	cmp = (t - s) - y;
 	s = t;
+}
+
+/*
+ * Energy terms of one lane's 16 elements (a, b0 + bb, c), bb = 0 .. 15, in
+ * bb order, added to (esum, ecmp): the disconnected term Z, the
+ * denominator Dn = d_ia + d_jb + d_kc and W (W - Z) / Dn, or for two W sets
+ * (W_L - Z) W_R / Dn (src/pt.c:700,927).  The loads that depend on a, c or
+ * (a, c) only are made once for the 16 elements; those that depend on b
+ * only are wave-uniform (b0 is); b is clamped into the arrays, so every
+ * element's loads are made whether or not the triangle keeps it, and only
+ * the sum is masked.  W values: ew[64 bb] (one set; the right set with two)
+ * and ew[64 (bb + wl_ofs)] (the left set).
+ */
+template <int F, int KIND, int NSET>
+__device__ __forceinline__ void
+epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
+    const typename Form<F>::AE *ew, int wl_ofs, double &esum, double &ecmp)
+{
+	typedef typename Form<F>::ER ER;
+	const int na = p.na;
+	const ER s0 = (ER)p.inv_scale[0], s1 = (ER)p.inv_scale[1];
+	const ER *d1 = (const ER *)p.epi.d1;
+	/* this element's term from its Z and Dn */
+	auto term = [&](int bb, ER z, ER dn) {
+		if constexpr (NSET == 1) {
+			ER w = (ER)ew[64 * bb];
+			if (Form<F>::SPLIT)
+				w *= s0;
+			return w * (w - z) / dn;
+		} else {
+			ER wl = (ER)ew[64 * (bb + wl_ofs)], wr = (ER)ew[64 * bb];
+			if (Form<F>::SPLIT) {
+				wl *= s0;
+				wr *= s1;
+			}
+			return (wl - z) * wr / dn;
+		}
+	};
+	if constexpr (KIND == PT_AAA) {
+		/* src/pt.c:162-175, for (u, G) = (t1 or l1, i_oovv) and
+		 * (f, Z) = (f_ov or f2_ov, t2 or l2) */
+		const int64_t o = p.oa, v = p.va, vv = v * v;
+		const ER *u = (const ER *)p.epi.u1, *f = (const ER *)p.epi.f1;
+		const ER *G = (const ER *)p.epi.GA, *Z = (const ER *)p.epi.ZA;
+		const int64_t jk = ((int64_t)j * o + k) * vv;
+		const int64_t ik = ((int64_t)i * o + k) * vv;
+		const int64_t ji = ((int64_t)j * o + i) * vv;
+		const ER uia = u[i * v + a], uja = u[j * v + a], uka = u[k * v + a];
+		const ER fia = f[i * v + a], fja = f[j * v + a], fka = f[k * v + a];
+		const ER uic = u[i * v + c], ujc = u[j * v + c], ukc = u[k * v + c];
+		const ER fic = f[i * v + c], fjc = f[j * v + c], fkc = f[k * v + c];
+		const int64_t ac = (int64_t)a * v + c;
+		const ER gjkac = G[jk + ac], gikac = G[ik + ac], gjiac = G[ji + ac];
+		const ER zjkac = Z[jk + ac], zikac = Z[ik + ac], zjiac = Z[ji + ac];
+		const ER dia = d1[i * v + a], dkc = d1[k * v + c];
+#pragma unroll 1
+		for (int bb = 0; bb < 16; ++bb) {
+			const int b = b0 + bb;
+			const int bq = b < na ? b : na - 1;
+			const int64_t bc = (int64_t)bq * v + c;
+			const int64_t ba = (int64_t)bq * v + a;
+			const ER uib = u[i * v + bq], ujb = u[j * v + bq];
+			const ER ukb = u[k * v + bq];
+			const ER fib = f[i * v + bq], fjb = f[j * v + bq];
+			const ER fkb = f[k * v + bq];
+			const ER zu = uia * G[jk + bc] - uja * G[ik + bc] -
+			    uka * G[ji + bc] - uib * gjkac + ujb * gikac +
+			    ukb * gjiac - uic * G[jk + ba] + ujc * G[ik + ba] +
+			    ukc * G[ji + ba];
+			const ER zf = fia * Z[jk + bc] - fja * Z[ik + bc] -
+			    fka * Z[ji + bc] - fib * zjkac + fjb * zikac +
+			    fkb * zjiac - fic * Z[jk + ba] + fjc * Z[ik + ba] +
+			    fkc * Z[ji + ba];
+			const ER t = term(bb, zu + zf, dia + d1[j * v + bq] + dkc);
+			if (a > b && b > c)
+				kahan(esum, ecmp, (double)t);
+		}
+	} else {
+		/* src/pt.c:177-185 and :433-442 */
+		const int64_t oa = p.oa, va = p.va, ob = p.ob, vb = p.vb;
+		const ER *u1 = (const ER *)p.epi.u1, *u2 = (const ER *)p.epi.u2;
+		const ER *f1 = (const ER *)p.epi.f1, *f2 = (const ER *)p.epi.f2;
+		const ER *GA = (const ER *)p.epi.GA, *GB = (const ER *)p.epi.GB;
+		const ER *ZA = (const ER *)p.epi.ZA, *ZB = (const ER *)p.epi.ZB;
+		const ER *d2 = (const ER *)p.epi.d2;
+		const int64_t jk = ((int64_t)j * ob + k) * va;
+		const int64_t ik = ((int64_t)i * ob + k) * va;
+		const int64_t ji = ((int64_t)j * oa + i) * va;
+		const ER uia = u1[i * va + a], fia = f1[i * va + a];
+		const ER uja = u1[j * va + a], fja = f1[j * va + a];
+		const ER u2c = u2[k * vb + c], f2c = f2[k * vb + c];
+		const int64_t jkac = (jk + a) * vb + c, ikac = (ik + a) * vb + c;
+		const ER gjkac = GB[jkac], zjkac = ZB[jkac];
+		const ER gikac = GB[ikac], zikac = ZB[ikac];
+		const ER dia = d1[i * va + a], dkc = d2[k * vb + c];
+#pragma unroll 1
+		for (int bb = 0; bb < 16; ++bb) {
+			const int b = b0 + bb;
+			const int bq = b < na ? b : na - 1;
+			const int64_t jkbc = (jk + bq) * vb + c;
+			const int64_t ikbc = (ik + bq) * vb + c;
+			const int64_t jiba = (ji + bq) * va + a;
+			const ER uib = u1[i * va + bq], fib = f1[i * va + bq];
+			const ER ujb = u1[j * va + bq], fjb = f1[j * va + bq];
+			const ER z = (uia * GB[jkbc] + fia * ZB[jkbc])
+			    - (uib * gjkac + fib * zjkac)
+			    - (uja * GB[ikbc] + fja * ZB[ikbc])
+			    + (ujb * gikac + fjb * zikac)
+			    + (u2c * GA[jiba] + f2c * ZA[jiba]);
+			const ER t = term(bb, z, dia + d1[j * va + bq] + dkc);
+			if (a > b)
+				kahan(esum, ecmp, (double)t);
+		}
+	}
 }
 
 /* ------------------------------------------------- one staging step */
@@ -756,7 +828,6 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	typedef typename FM::G G;
 	typedef typename FM::AE AE;
 	typedef typename FM::ACC ACC;
-	typedef typename FM::ER ER;
 	constexpr int KC = FM::KC;
 	constexpr int TC = GM::TC, CW = GM::CW;
 	constexpr int NCB = CW / 16;
@@ -1023,6 +1094,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	 * lane's own LDS slots so the element loop runs with a runtime index. */
 	double esum = 0.0, ecmp = 0.0;
 	AE *E = sX + wave * (NSET * CW * 64) + lane;
+	const int b0u = uni(b_lo);  /* the wave's first b */
 #pragma unroll
 	for (int r = 0; r < 4; ++r) {
 		__syncthreads();
@@ -1039,41 +1111,12 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		const int a = a_lo + arow;
 		if (a >= na)
 			continue;
-		for (int x = 0; x < CW; ++x) {
-			const int b = b_lo + (x & 15);
-			const int c = c_lo + 16 * (x >> 4) + ln;
-			if (!(c < nc && a > b && (!same || b > c)))
-				continue;
-			ER z, dn;
-			const ER *d1 = (const ER *)p.epi.d1;
-			if (same) {
-				const int64_t o = p.oa, v = p.va;
-				z = z_same<ER>((const ER *)p.epi.u1, (const ER *)p.epi.GA,
-				    o, v, i, j, k, a, b, c) +
-				    z_same<ER>((const ER *)p.epi.f1, (const ER *)p.epi.ZA,
-				    o, v, i, j, k, a, b, c);
-				dn = d1[i * v + a] + d1[j * v + b] + d1[k * v + c];
-			} else {
-				const ER *d2 = (const ER *)p.epi.d2;
-				z = z_mixed<ER>(p.epi, p.oa, p.va, p.ob, p.vb, i, j, k, a, b,
-				    c);
-				dn = d1[i * p.va + a] + d1[j * p.va + b] + d2[k * p.vb + c];
-			}
-			ER term;
-			if (NSET == 1) {
-				ER w = (ER)E[x * 64];
-				if (FM::SPLIT)
-					w *= (ER)p.inv_scale[0];
-				term = w * (w - z) / dn;
-			} else {
-				ER wl = (ER)E[(CW + x) * 64], wr = (ER)E[x * 64];
-				if (FM::SPLIT) {
-					wl *= (ER)p.inv_scale[0];
-					wr *= (ER)p.inv_scale[1];
-				}
-				term = (wl - z) * wr / dn;
-			}
-			kahan(esum, ecmp, (double)term);
+#pragma unroll 1
+		for (int cb = 0; cb < NCB; ++cb) {
+			const int c = c_lo + 16 * cb + ln;
+			if (c < nc)
+				epi_block<F, KIND, NSET>(p, i, j, k, a, c, b0u,
+				    E + 16 * cb * 64, CW, esum, ecmp);
 		}
 	}
 
