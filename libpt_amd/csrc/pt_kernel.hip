@@ -160,12 +160,16 @@ kahan(double &s, double &cmp, double x)
  * only are wave-uniform (b0 is); b is clamped into the arrays, so every
  * element's loads are made whether or not the triangle keeps it, and only
  * the sum is masked.  W values: ew[64 bb] (one set; the right set with two)
- * and ew[64 (bb + wl_ofs)] (the left set).
+ * and ew[64 (bb + wl_ofs)] (the left set).  STC > 0 (mixed spin): the
+ * workgroup has staged the (b, c) values of i_oovv and t2 for jk and ik in
+ * LDS, sbc[(32 n + bb) STC] for n = GB jk, ZB jk, GB ik, ZB ik (sbc points
+ * at this lane's b0 and c).
  */
-template <int F, int KIND, int NSET>
+template <int F, int KIND, int NSET, int STC>
 __device__ __forceinline__ void
 epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
-    const typename Form<F>::AE *ew, int wl_ofs, double &esum, double &ecmp)
+    const typename Form<F>::AE *ew, int wl_ofs,
+    const typename Form<F>::ER *sbc, double &esum, double &ecmp)
 {
 	typedef typename Form<F>::ER ER;
 	const int na = p.na;
@@ -253,9 +257,21 @@ epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
 			const int64_t jiba = (ji + bq) * va + a;
 			const ER uib = u1[i * va + bq], fib = f1[i * va + bq];
 			const ER ujb = u1[j * va + bq], fjb = f1[j * va + bq];
-			const ER z = (uia * GB[jkbc] + fia * ZB[jkbc])
+			ER gjkbc, zjkbc, gikbc, zikbc;
+			if constexpr (STC > 0) {
+				gjkbc = sbc[bb * STC];
+				zjkbc = sbc[(32 + bb) * STC];
+				gikbc = sbc[(64 + bb) * STC];
+				zikbc = sbc[(96 + bb) * STC];
+			} else {
+				gjkbc = GB[jkbc];
+				zjkbc = ZB[jkbc];
+				gikbc = GB[ikbc];
+				zikbc = ZB[ikbc];
+			}
+			const ER z = (uia * gjkbc + fia * zjkbc)
 			    - (uib * gjkac + fib * zjkac)
-			    - (uja * GB[ikbc] + fja * ZB[ikbc])
+			    - (uja * gikbc + fja * zikbc)
 			    + (ujb * gikac + fjb * zikac)
 			    + (u2c * GA[jiba] + f2c * ZA[jiba]);
 			const ER t = term(bb, z, dia + d1[j * va + bq] + dkc);
@@ -1095,6 +1111,33 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	double esum = 0.0, ecmp = 0.0;
 	AE *E = sX + wave * (NSET * CW * 64) + lane;
 	const int b0u = uni(b_lo);  /* the wave's first b */
+	/* mixed spin: the tile's (b, c) blocks of i_oovv and t2 for jk and ik,
+	 * [GB jk, ZB jk, GB ik, ZB ik][32 b][TC c], staged once in LDS behind
+	 * the W values (rows and columns beyond the arrays repeat the last one;
+	 * the epilogue masks those elements).  The previous step's barrier
+	 * ended every read of the slots, and the barrier of the first W
+	 * column below orders these writes before the reads. */
+	typedef typename FM::ER ER;
+	constexpr int EBYTES = NW * NSET * CW * 64 * (int)sizeof(AE);
+	constexpr int NBC = 4 * 32 * TC;
+	constexpr bool STG = KIND == PT_AAB &&
+	    EBYTES + NBC * (int)sizeof(ER) <= SMEM && NBC % NT == 0;
+	const ER *sbc = (const ER *)(smem + EBYTES) + 16 * wb * TC + wc + ln;
+	if constexpr (STG) {
+		ER *dst = (ER *)(smem + EBYTES);
+		const ER *GB = (const ER *)p.epi.GB, *ZB = (const ER *)p.epi.ZB;
+		const int64_t jk = ((int64_t)j * p.ob + k) * p.va;
+		const int64_t ik = ((int64_t)i * p.ob + k) * p.va;
+#pragma unroll
+		for (int q = 0; q < NBC / NT; ++q) {
+			const int n = tid + q * NT;
+			const int arr = n / (32 * TC), bl = (n / TC) % 32, cl = n % TC;
+			const int b = B0 + bl < na ? B0 + bl : na - 1;
+			const int c = C0 + cl < nc ? C0 + cl : nc - 1;
+			dst[n] = ((arr & 1) ? ZB : GB)[((arr < 2 ? jk : ik) + b) *
+			    p.vb + c];
+		}
+	}
 #pragma unroll
 	for (int r = 0; r < 4; ++r) {
 		__syncthreads();
@@ -1115,8 +1158,8 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		for (int cb = 0; cb < NCB; ++cb) {
 			const int c = c_lo + 16 * cb + ln;
 			if (c < nc)
-				epi_block<F, KIND, NSET>(p, i, j, k, a, c, b0u,
-				    E + 16 * cb * 64, CW, esum, ecmp);
+				epi_block<F, KIND, NSET, STG ? TC : 0>(p, i, j, k, a, c,
+				    b0u, E + 16 * cb * 64, CW, sbc + 16 * cb, esum, ecmp);
 		}
 	}
 
