@@ -161,15 +161,19 @@ kahan(double &s, double &cmp, double x)
  * element's loads are made whether or not the triangle keeps it, and only
  * the sum is masked.  W values: ew[64 bb] (one set; the right set with two)
  * and ew[64 (bb + wl_ofs)] (the left set).  STC > 0 (mixed spin): the
- * workgroup has staged the (b, c) values of i_oovv and t2 for jk and ik in
- * LDS, sbc[(32 n + bb) STC] for n = GB jk, ZB jk, GB ik, ZB ik (sbc points
- * at this lane's b0 and c).
+ * workgroup has staged in LDS the (b, c) values of i_oovv and t2 for jk
+ * and ik, sbc[(32 n + bb) STC] for n = GB jk, ZB jk, GB ik, ZB ik (sbc
+ * points at this lane's b0 and c), and the values that depend on b only,
+ * sbv[32 n + bb] for n = u1 i, f1 i, u1 j, f1 j, d1 j (sbv points at b0);
+ * SBA: also the (b, a) values of the same-spin blocks for ji,
+ * sba[(32 n + bb) 32] for n = GA, ZA (sba points at b0 and this lane's a).
  */
-template <int F, int KIND, int NSET, int STC>
+template <int F, int KIND, int NSET, int STC, bool SBA>
 __device__ __forceinline__ void
 epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
     const typename Form<F>::AE *ew, int wl_ofs,
-    const typename Form<F>::ER *sbc, double &esum, double &ecmp)
+    const typename Form<F>::ER *sbc, const typename Form<F>::ER *sba,
+    const typename Form<F>::ER *sbv, double &esum, double &ecmp)
 {
 	typedef typename Form<F>::ER ER;
 	const int na = p.na;
@@ -255,26 +259,42 @@ epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
 			const int64_t jkbc = (jk + bq) * vb + c;
 			const int64_t ikbc = (ik + bq) * vb + c;
 			const int64_t jiba = (ji + bq) * va + a;
-			const ER uib = u1[i * va + bq], fib = f1[i * va + bq];
-			const ER ujb = u1[j * va + bq], fjb = f1[j * va + bq];
-			ER gjkbc, zjkbc, gikbc, zikbc;
+			ER uib, fib, ujb, fjb, djb;
+			ER gjkbc, zjkbc, gikbc, zikbc, gjiba, zjiba;
 			if constexpr (STC > 0) {
+				uib = sbv[bb];
+				fib = sbv[32 + bb];
+				ujb = sbv[64 + bb];
+				fjb = sbv[96 + bb];
+				djb = sbv[128 + bb];
 				gjkbc = sbc[bb * STC];
 				zjkbc = sbc[(32 + bb) * STC];
 				gikbc = sbc[(64 + bb) * STC];
 				zikbc = sbc[(96 + bb) * STC];
 			} else {
+				uib = u1[i * va + bq];
+				fib = f1[i * va + bq];
+				ujb = u1[j * va + bq];
+				fjb = f1[j * va + bq];
+				djb = d1[j * va + bq];
 				gjkbc = GB[jkbc];
 				zjkbc = ZB[jkbc];
 				gikbc = GB[ikbc];
 				zikbc = ZB[ikbc];
 			}
+			if constexpr (SBA) {
+				gjiba = sba[bb * 32];
+				zjiba = sba[(32 + bb) * 32];
+			} else {
+				gjiba = GA[jiba];
+				zjiba = ZA[jiba];
+			}
 			const ER z = (uia * gjkbc + fia * zjkbc)
 			    - (uib * gjkac + fib * zjkac)
 			    - (uja * gikbc + fja * zikbc)
 			    + (ujb * gikac + fjb * zikac)
-			    + (u2c * GA[jiba] + f2c * ZA[jiba]);
-			const ER t = term(bb, z, dia + d1[j * va + bq] + dkc);
+			    + (u2c * gjiba + f2c * zjiba);
+			const ER t = term(bb, z, dia + djb + dkc);
 			if (a > b)
 				kahan(esum, ecmp, (double)t);
 		}
@@ -1111,20 +1131,28 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	double esum = 0.0, ecmp = 0.0;
 	AE *E = sX + wave * (NSET * CW * 64) + lane;
 	const int b0u = uni(b_lo);  /* the wave's first b */
-	/* mixed spin: the tile's (b, c) blocks of i_oovv and t2 for jk and ik,
-	 * [GB jk, ZB jk, GB ik, ZB ik][32 b][TC c], staged once in LDS behind
-	 * the W values (rows and columns beyond the arrays repeat the last one;
+	/* mixed spin: the tile's epilogue operands that do not depend on a or
+	 * c alone, staged once in LDS behind the W values: the (b, c) blocks
+	 * of i_oovv and t2 for jk and ik, [GB jk, ZB jk, GB ik, ZB ik][32 b]
+	 * [TC c], the b vectors [u1 i, f1 i, u1 j, f1 j, d1 j][32 b] and, where
+	 * LDS allows, the (b, a) blocks of the same-spin ones for ji, [GA, ZA]
+	 * [32 b][32 a] (rows and columns beyond the arrays repeat the last one;
 	 * the epilogue masks those elements).  The previous step's barrier
 	 * ended every read of the slots, and the barrier of the first W
 	 * column below orders these writes before the reads. */
 	typedef typename FM::ER ER;
 	constexpr int EBYTES = NW * NSET * CW * 64 * (int)sizeof(AE);
-	constexpr int NBC = 4 * 32 * TC;
+	constexpr int NBC = 4 * 32 * TC, NBV = 5 * 32, NBA = 2 * 32 * 32;
+	constexpr int ES_ = (int)sizeof(ER);
 	constexpr bool STG = KIND == PT_AAB &&
-	    EBYTES + NBC * (int)sizeof(ER) <= SMEM && NBC % NT == 0;
-	const ER *sbc = (const ER *)(smem + EBYTES) + 16 * wb * TC + wc + ln;
+	    EBYTES + (NBC + NBV) * ES_ <= SMEM && NBC % NT == 0;
+	constexpr bool SBA = STG && EBYTES + (NBC + NBV + NBA) * ES_ <= SMEM &&
+	    NBA % NT == 0;
+	ER *const stg = (ER *)(smem + EBYTES);
+	const ER *sbc = stg + 16 * wb * TC + wc + ln;
+	const ER *sbv = stg + NBC + 16 * wb;
+	const ER *sba = stg + NBC + NBV + 16 * wb * 32 + 16 * wa;
 	if constexpr (STG) {
-		ER *dst = (ER *)(smem + EBYTES);
 		const ER *GB = (const ER *)p.epi.GB, *ZB = (const ER *)p.epi.ZB;
 		const int64_t jk = ((int64_t)j * p.ob + k) * p.va;
 		const int64_t ik = ((int64_t)i * p.ob + k) * p.va;
@@ -1134,8 +1162,27 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 			const int arr = n / (32 * TC), bl = (n / TC) % 32, cl = n % TC;
 			const int b = B0 + bl < na ? B0 + bl : na - 1;
 			const int c = C0 + cl < nc ? C0 + cl : nc - 1;
-			dst[n] = ((arr & 1) ? ZB : GB)[((arr < 2 ? jk : ik) + b) *
+			stg[n] = ((arr & 1) ? ZB : GB)[((arr < 2 ? jk : ik) + b) *
 			    p.vb + c];
+		}
+		if (tid < NBV) {
+			const int arr = tid / 32, bl = tid % 32;
+			const int b = B0 + bl < na ? B0 + bl : na - 1;
+			const ER *src = arr == 4 ? (const ER *)p.epi.d1 :
+			    (arr & 1) ? (const ER *)p.epi.f1 : (const ER *)p.epi.u1;
+			stg[NBC + tid] = src[(arr < 2 ? i : j) * p.va + b];
+		}
+	}
+	if constexpr (SBA) {
+		const ER *GA = (const ER *)p.epi.GA, *ZA = (const ER *)p.epi.ZA;
+		const int64_t ji = ((int64_t)j * p.oa + i) * p.va;
+#pragma unroll
+		for (int q = 0; q < NBA / NT; ++q) {
+			const int n = tid + q * NT;
+			const int arr = n / 1024, bl = (n / 32) % 32, al = n % 32;
+			const int b = B0 + bl < na ? B0 + bl : na - 1;
+			const int a = A0 + al < na ? A0 + al : na - 1;
+			stg[NBC + NBV + n] = (arr ? ZA : GA)[(ji + b) * p.va + a];
 		}
 	}
 #pragma unroll
@@ -1158,8 +1205,9 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		for (int cb = 0; cb < NCB; ++cb) {
 			const int c = c_lo + 16 * cb + ln;
 			if (c < nc)
-				epi_block<F, KIND, NSET, STG ? TC : 0>(p, i, j, k, a, c,
-				    b0u, E + 16 * cb * 64, CW, sbc + 16 * cb, esum, ecmp);
+				epi_block<F, KIND, NSET, STG ? TC : 0, SBA>(p, i, j, k,
+				    a, c, b0u, E + 16 * cb * 64, CW, sbc + 16 * cb,
+				    sba + arow, sbv, esum, ecmp);
 		}
 	}
 
