@@ -870,7 +870,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	constexpr int NT = GM::THREADS;
 	constexpr int GB = 32;  /* bytes per packed kappa group and pair */
 	constexpr int SMEM = SG::BYTES;
-	static_assert(SMEM >= NW * NSET * CW * 64 * (int)sizeof(AE),
+	static_assert(SMEM >= NW * NSET * 16 * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
 	static_assert(SG::SLOT >= 4 * 4 * 16 * 17 * (int)sizeof(AE),
 	    "transpose round exceeds a slot");
@@ -1126,10 +1126,11 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	}
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
-	 * c = C0 + wc + 16 cb + ln.  Each register column r goes through this
-	 * lane's own LDS slots so the element loop runs with a runtime index. */
+	 * c = C0 + wc + 16 cb + ln.  Each register column r of each 16-wide
+	 * c block goes through this lane's own LDS slots (16 per W set) so
+	 * the element loop runs with a runtime index. */
 	double esum = 0.0, ecmp = 0.0;
-	AE *E = sX + wave * (NSET * CW * 64) + lane;
+	AE *E = sX + wave * (NSET * 16 * 64) + lane;
 	const int b0u = uni(b_lo);  /* the wave's first b */
 	/* mixed spin: the tile's epilogue operands that do not depend on a or
 	 * c alone, staged once in LDS behind the W values: the (b, c) blocks
@@ -1141,7 +1142,7 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	 * ended every read of the slots, and the barrier of the first W
 	 * column below orders these writes before the reads. */
 	typedef typename FM::ER ER;
-	constexpr int EBYTES = NW * NSET * CW * 64 * (int)sizeof(AE);
+	constexpr int EBYTES = NW * NSET * 16 * 64 * (int)sizeof(AE);
 	constexpr int NBC = 4 * 32 * TC, NBV = 5 * 32, NBA = 2 * 32 * 32;
 	constexpr int ES_ = (int)sizeof(ER);
 	constexpr bool STG = KIND == PT_AAB &&
@@ -1186,29 +1187,25 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		}
 	}
 #pragma unroll
-	for (int r = 0; r < 4; ++r) {
+	for (int r = 0; r < 4; ++r)
+#pragma unroll
+	for (int cb = 0; cb < NCB; ++cb) {
 		__syncthreads();
 #pragma unroll
-		for (int x = 0; x < CW; ++x) {
-			E[x * 64] = acc[x][r];
+		for (int x = 0; x < 16; ++x) {
+			E[x * 64] = acc[16 * cb + x][r];
 			if (NSET == 2)
-				E[(CW + x) * 64] = bload_ae<AE>(wr, wvo,
-				    (r * CW + x) * WST);
+				E[(16 + x) * 64] = bload_ae<AE>(wr, wvo,
+				    (r * CW + 16 * cb + x) * WST);
 		}
 		if (!act)
 			continue;
 		const int arow = FM::ROW_F64 ? (lg + 4 * r) : (4 * lg + r);
 		const int a = a_lo + arow;
-		if (a >= na)
-			continue;
-#pragma unroll 1
-		for (int cb = 0; cb < NCB; ++cb) {
-			const int c = c_lo + 16 * cb + ln;
-			if (c < nc)
-				epi_block<F, KIND, NSET, STG ? TC : 0, SBA>(p, i, j, k,
-				    a, c, b0u, E + 16 * cb * 64, CW, sbc + 16 * cb,
-				    sba + arow, sbv, esum, ecmp);
-		}
+		const int c = c_lo + 16 * cb + ln;
+		if (a < na && c < nc)
+			epi_block<F, KIND, NSET, STG ? TC : 0, SBA>(p, i, j, k, a, c,
+			    b0u, E, 16, sbc + 16 * cb, sba + arow, sbv, esum, ecmp);
 	}
 
 	/* deterministic workgroup reduction */
