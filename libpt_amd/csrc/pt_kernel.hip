@@ -160,13 +160,14 @@ kahan(double &s, double &cmp, double x)
  * only are wave-uniform (b0 is); b is clamped into the arrays, so every
  * element's loads are made whether or not the triangle keeps it, and only
  * the sum is masked.  W values: ew[64 bb] (one set; the right set with two)
- * and ew[64 (bb + wl_ofs)] (the left set).  STC > 0 (mixed spin): the
- * workgroup has staged in LDS the (b, c) values of i_oovv and t2 for jk
- * and ik, sbc[(32 n + bb) STC] for n = GB jk, ZB jk, GB ik, ZB ik (sbc
- * points at this lane's b0 and c), and the values that depend on b only,
- * sbv[32 n + bb] for n = u1 i, f1 i, u1 j, f1 j, d1 j (sbv points at b0);
- * SBA: also the (b, a) values of the same-spin blocks for ji,
- * sba[(32 n + bb) 32] for n = GA, ZA (sba points at b0 and this lane's a).
+ * and ew[64 (bb + wl_ofs)] (the left set).  STC > 0: the workgroup has
+ * staged in LDS the (b, c) values of i_oovv and t2, sbc[(32 n + bb) STC]
+ * (sbc points at this lane's b0 and c) for n = G jk, Z jk, G ik, Z ik
+ * (mixed spin: GB, ZB; same spin also G ji, Z ji), and the values that
+ * depend on b only, sbv[32 n + bb] (sbv points at b0) for n = u i, f i,
+ * u j, f j, (same spin: u k, f k,) d1 j; SBA (mixed spin): also the (b, a)
+ * values of the same-spin blocks for ji, sba[(32 n + bb) 32] for n = GA,
+ * ZA (sba points at b0 and this lane's a).
  */
 template <int F, int KIND, int NSET, int STC, bool SBA>
 __device__ __forceinline__ void
@@ -218,19 +219,46 @@ epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
 			const int bq = b < na ? b : na - 1;
 			const int64_t bc = (int64_t)bq * v + c;
 			const int64_t ba = (int64_t)bq * v + a;
-			const ER uib = u[i * v + bq], ujb = u[j * v + bq];
-			const ER ukb = u[k * v + bq];
-			const ER fib = f[i * v + bq], fjb = f[j * v + bq];
-			const ER fkb = f[k * v + bq];
-			const ER zu = uia * G[jk + bc] - uja * G[ik + bc] -
-			    uka * G[ji + bc] - uib * gjkac + ujb * gikac +
+			ER uib, ujb, ukb, fib, fjb, fkb, djb;
+			ER gjkbc, zjkbc, gikbc, zikbc, gjibc, zjibc;
+			if constexpr (STC > 0) {
+				uib = sbv[bb];
+				fib = sbv[32 + bb];
+				ujb = sbv[64 + bb];
+				fjb = sbv[96 + bb];
+				ukb = sbv[128 + bb];
+				fkb = sbv[160 + bb];
+				djb = sbv[192 + bb];
+				gjkbc = sbc[bb * STC];
+				zjkbc = sbc[(32 + bb) * STC];
+				gikbc = sbc[(64 + bb) * STC];
+				zikbc = sbc[(96 + bb) * STC];
+				gjibc = sbc[(128 + bb) * STC];
+				zjibc = sbc[(160 + bb) * STC];
+			} else {
+				uib = u[i * v + bq];
+				fib = f[i * v + bq];
+				ujb = u[j * v + bq];
+				fjb = f[j * v + bq];
+				ukb = u[k * v + bq];
+				fkb = f[k * v + bq];
+				djb = d1[j * v + bq];
+				gjkbc = G[jk + bc];
+				zjkbc = Z[jk + bc];
+				gikbc = G[ik + bc];
+				zikbc = Z[ik + bc];
+				gjibc = G[ji + bc];
+				zjibc = Z[ji + bc];
+			}
+			const ER zu = uia * gjkbc - uja * gikbc -
+			    uka * gjibc - uib * gjkac + ujb * gikac +
 			    ukb * gjiac - uic * G[jk + ba] + ujc * G[ik + ba] +
 			    ukc * G[ji + ba];
-			const ER zf = fia * Z[jk + bc] - fja * Z[ik + bc] -
-			    fka * Z[ji + bc] - fib * zjkac + fjb * zikac +
+			const ER zf = fia * zjkbc - fja * zikbc -
+			    fka * zjibc - fib * zjkac + fjb * zikac +
 			    fkb * zjiac - fic * Z[jk + ba] + fjc * Z[ik + ba] +
 			    fkc * Z[ji + ba];
-			const ER t = term(bb, zu + zf, dia + d1[j * v + bq] + dkc);
+			const ER t = term(bb, zu + zf, dia + djb + dkc);
 			if (a > b && b > c)
 				kahan(esum, ecmp, (double)t);
 		}
@@ -1132,46 +1160,61 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	double esum = 0.0, ecmp = 0.0;
 	AE *E = sX + wave * (NSET * 16 * 64) + lane;
 	const int b0u = uni(b_lo);  /* the wave's first b */
-	/* mixed spin: the tile's epilogue operands that do not depend on a or
-	 * c alone, staged once in LDS behind the W values: the (b, c) blocks
-	 * of i_oovv and t2 for jk and ik, [GB jk, ZB jk, GB ik, ZB ik][32 b]
-	 * [TC c], the b vectors [u1 i, f1 i, u1 j, f1 j, d1 j][32 b] and, where
-	 * LDS allows, the (b, a) blocks of the same-spin ones for ji, [GA, ZA]
-	 * [32 b][32 a] (rows and columns beyond the arrays repeat the last one;
-	 * the epilogue masks those elements).  The previous step's barrier
-	 * ended every read of the slots, and the barrier of the first W
-	 * column below orders these writes before the reads. */
+	/* the tile's epilogue operands that do not depend on a or c alone,
+	 * staged once in LDS behind the W values where they fit: the (b, c)
+	 * blocks of i_oovv and t2, [G jk, Z jk, G ik, Z ik (, G ji, Z ji)]
+	 * [32 b][TC c] (mixed spin: the GB, ZB blocks; same spin: GA, ZA with
+	 * ji), the b vectors [u i, f i, u j, f j, (u k, f k,) d1 j][32 b] and,
+	 * for the mixed spin case where LDS allows, the (b, a) blocks of the
+	 * same-spin ones for ji, [GA, ZA][32 b][32 a] (rows and columns beyond
+	 * the arrays repeat the last one; the epilogue masks those elements).
+	 * The previous step's barrier ended every read of the slots, and the
+	 * barrier of the first W column below orders these writes before the
+	 * reads. */
 	typedef typename FM::ER ER;
 	constexpr int EBYTES = NW * NSET * 16 * 64 * (int)sizeof(AE);
-	constexpr int NBC = 4 * 32 * TC, NBV = 5 * 32, NBA = 2 * 32 * 32;
+	constexpr int NPB = same ? 6 : 4;     /* (b, c) blocks */
+	constexpr int NBC = NPB * 32 * TC, NBV = (NPB + 1) * 32;
+	constexpr int NBA = 2 * 32 * 32;
 	constexpr int ES_ = (int)sizeof(ER);
-	constexpr bool STG = KIND == PT_AAB &&
-	    EBYTES + (NBC + NBV) * ES_ <= SMEM && NBC % NT == 0;
-	constexpr bool SBA = STG && EBYTES + (NBC + NBV + NBA) * ES_ <= SMEM &&
-	    NBA % NT == 0;
+	constexpr bool STG = EBYTES + (NBC + NBV) * ES_ <= SMEM &&
+	    NBC % NT == 0 && NBV <= NT;
+	constexpr bool SBA = !same && STG && NBA % NT == 0 &&
+	    EBYTES + (NBC + NBV + NBA) * ES_ <= SMEM;
 	ER *const stg = (ER *)(smem + EBYTES);
 	const ER *sbc = stg + 16 * wb * TC + wc + ln;
 	const ER *sbv = stg + NBC + 16 * wb;
 	const ER *sba = stg + NBC + NBV + 16 * wb * 32 + 16 * wa;
 	if constexpr (STG) {
-		const ER *GB = (const ER *)p.epi.GB, *ZB = (const ER *)p.epi.ZB;
-		const int64_t jk = ((int64_t)j * p.ob + k) * p.va;
-		const int64_t ik = ((int64_t)i * p.ob + k) * p.va;
+		/* block n: (G, Z)[n & 1] of pair n / 2 = jk, ik (, ji); mixed
+		 * spin: rows (jk + b) vb, same spin: jk + b v, v = vb */
+		const ER *G = (const ER *)(same ? p.epi.GA : p.epi.GB);
+		const ER *Z = (const ER *)(same ? p.epi.ZA : p.epi.ZB);
+		const int64_t po = same ? p.oa : p.ob;
+		const int64_t pr = same ? p.va * p.va : p.va;
+		const int64_t pjk = ((int64_t)j * po + k) * pr;
+		const int64_t pik = ((int64_t)i * po + k) * pr;
+		const int64_t pji = ((int64_t)j * p.oa + i) * pr;
 #pragma unroll
 		for (int q = 0; q < NBC / NT; ++q) {
 			const int n = tid + q * NT;
 			const int arr = n / (32 * TC), bl = (n / TC) % 32, cl = n % TC;
 			const int b = B0 + bl < na ? B0 + bl : na - 1;
 			const int c = C0 + cl < nc ? C0 + cl : nc - 1;
-			stg[n] = ((arr & 1) ? ZB : GB)[((arr < 2 ? jk : ik) + b) *
-			    p.vb + c];
+			const int64_t pb = arr < 2 ? pjk : arr < 4 ? pik : pji;
+			const int64_t row = same ? pb + (int64_t)b * p.vb :
+			    (pb + b) * p.vb;
+			stg[n] = ((arr & 1) ? Z : G)[row + c];
 		}
+		/* vector n: (u, f)[n & 1] at occupied index (i, j, k)[n / 2], the
+		 * last one d1 at j */
 		if (tid < NBV) {
 			const int arr = tid / 32, bl = tid % 32;
 			const int b = B0 + bl < na ? B0 + bl : na - 1;
-			const ER *src = arr == 4 ? (const ER *)p.epi.d1 :
+			const int occ = arr < 2 ? i : arr < 4 || arr == NPB ? j : k;
+			const ER *src = arr == NPB ? (const ER *)p.epi.d1 :
 			    (arr & 1) ? (const ER *)p.epi.f1 : (const ER *)p.epi.u1;
-			stg[NBC + tid] = src[(arr < 2 ? i : j) * p.va + b];
+			stg[NBC + tid] = src[occ * p.va + b];
 		}
 	}
 	if constexpr (SBA) {
