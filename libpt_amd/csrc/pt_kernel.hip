@@ -152,11 +152,11 @@ kahan(double &s, double &cmp, double x)
 }
 
 /*
- * Energy terms of one lane's 16 elements (a, b0 + bb, c), bb = 0 .. 15, in
- * bb order, added to (esum, ecmp): the disconnected term Z, the
+ * Energy terms of one lane's NB elements (a, b0 + bb, c), bb = 0 .. NB-1,
+ * in bb order, added to (esum, ecmp): the disconnected term Z, the
  * denominator Dn = d_ia + d_jb + d_kc and W (W - Z) / Dn, or for two W sets
  * (W_L - Z) W_R / Dn (src/pt.c:700,927).  The loads that depend on a, c or
- * (a, c) only are made once for the 16 elements; those that depend on b
+ * (a, c) only are made once for the NB elements; those that depend on b
  * only are wave-uniform (b0 is); b is clamped into the arrays, so every
  * element's loads are made whether or not the triangle keeps it, and only
  * the sum is masked.  W values: ew[64 bb] (one set; the right set with two)
@@ -169,7 +169,7 @@ kahan(double &s, double &cmp, double x)
  * values of the same-spin blocks for ji, sba[(32 n + bb) 32] for n = GA,
  * ZA (sba points at b0 and this lane's a).
  */
-template <int F, int KIND, int NSET, int STC, bool SBA>
+template <int F, int KIND, int NSET, int NB, int STC, bool SBA>
 __device__ __forceinline__ void
 epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
     const typename Form<F>::AE *ew, int wl_ofs,
@@ -214,7 +214,7 @@ epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
 		const ER zjkac = Z[jk + ac], zikac = Z[ik + ac], zjiac = Z[ji + ac];
 		const ER dia = d1[i * v + a], dkc = d1[k * v + c];
 #pragma unroll 1
-		for (int bb = 0; bb < 16; ++bb) {
+		for (int bb = 0; bb < NB; ++bb) {
 			const int b = b0 + bb;
 			const int bq = b < na ? b : na - 1;
 			const int64_t bc = (int64_t)bq * v + c;
@@ -281,7 +281,7 @@ epi_block(const pt_launch &p, int i, int j, int k, int a, int c, int b0,
 		const ER gikac = GB[ikac], zikac = ZB[ikac];
 		const ER dia = d1[i * va + a], dkc = d2[k * vb + c];
 #pragma unroll 1
-		for (int bb = 0; bb < 16; ++bb) {
+		for (int bb = 0; bb < NB; ++bb) {
 			const int b = b0 + bb;
 			const int bq = b < na ? b : na - 1;
 			const int64_t jkbc = (jk + bq) * vb + c;
@@ -898,7 +898,8 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	constexpr int NT = GM::THREADS;
 	constexpr int GB = 32;  /* bytes per packed kappa group and pair */
 	constexpr int SMEM = SG::BYTES;
-	static_assert(SMEM >= NW * NSET * 16 * 64 * (int)sizeof(AE),
+	static_assert(SMEM >= NW * 2 * 8 * 64 * (int)sizeof(AE) &&
+	    SMEM >= NW * 16 * 64 * (int)sizeof(AE),
 	    "epilogue slots exceed the stage area");
 	static_assert(SG::SLOT >= 4 * 4 * 16 * 17 * (int)sizeof(AE),
 	    "transpose round exceeds a slot");
@@ -1155,10 +1156,9 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 
 	/* epilogue: layout [a][c]@b, block x = 16 cb + b; lane holds
 	 * c = C0 + wc + 16 cb + ln.  Each register column r of each 16-wide
-	 * c block goes through this lane's own LDS slots (16 per W set) so
-	 * the element loop runs with a runtime index. */
+	 * c block goes through this lane's own LDS slots (BH b per W set at a
+	 * time) so the element loop runs with a runtime index. */
 	double esum = 0.0, ecmp = 0.0;
-	AE *E = sX + wave * (NSET * 16 * 64) + lane;
 	const int b0u = uni(b_lo);  /* the wave's first b */
 	/* the tile's epilogue operands that do not depend on a or c alone,
 	 * staged once in LDS behind the W values where they fit: the (b, c)
@@ -1172,11 +1172,16 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 	 * barrier of the first W column below orders these writes before the
 	 * reads. */
 	typedef typename FM::ER ER;
-	constexpr int EBYTES = NW * NSET * 16 * 64 * (int)sizeof(AE);
 	constexpr int NPB = same ? 6 : 4;     /* (b, c) blocks */
 	constexpr int NBC = NPB * 32 * TC, NBV = (NPB + 1) * 32;
 	constexpr int NBA = 2 * 32 * 32;
 	constexpr int ES_ = (int)sizeof(ER);
+	/* W passes of BH b each: 8 where W of 16 b (both sets) would leave no
+	 * room for the staged operands */
+	constexpr int BH = NW * NSET * 16 * 64 * (int)sizeof(AE) +
+	    (NBC + NBV + (same ? 0 : NBA)) * ES_ <= SMEM ? 16 : 8;
+	constexpr int EBYTES = NW * NSET * BH * 64 * (int)sizeof(AE);
+	AE *E = sX + wave * (NSET * BH * 64) + lane;
 	constexpr bool STG = EBYTES + (NBC + NBV) * ES_ <= SMEM &&
 	    NBC % NT == 0 && NBV <= NT;
 	constexpr bool SBA = !same && STG && NBA % NT == 0 &&
@@ -1232,14 +1237,17 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 #pragma unroll
 	for (int r = 0; r < 4; ++r)
 #pragma unroll
-	for (int cb = 0; cb < NCB; ++cb) {
+	for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+	for (int h = 0; h < 16 / BH; ++h) {
+		const int x0 = 16 * cb + BH * h;
 		__syncthreads();
 #pragma unroll
-		for (int x = 0; x < 16; ++x) {
-			E[x * 64] = acc[16 * cb + x][r];
+		for (int x = 0; x < BH; ++x) {
+			E[x * 64] = acc[x0 + x][r];
 			if (NSET == 2)
-				E[(16 + x) * 64] = bload_ae<AE>(wr, wvo,
-				    (r * CW + 16 * cb + x) * WST);
+				E[(BH + x) * 64] = bload_ae<AE>(wr, wvo,
+				    (r * CW + x0 + x) * WST);
 		}
 		if (!act)
 			continue;
@@ -1247,8 +1255,9 @@ tile_item(const pt_launch &p, int64_t it, typename Form<F>::AE *wsv)
 		const int a = a_lo + arow;
 		const int c = c_lo + 16 * cb + ln;
 		if (a < na && c < nc)
-			epi_block<F, KIND, NSET, STG ? TC : 0, SBA>(p, i, j, k, a, c,
-			    b0u, E, 16, sbc + 16 * cb, sba + arow, sbv, esum, ecmp);
+			epi_block<F, KIND, NSET, BH, STG ? TC : 0, SBA>(p, i, j, k, a,
+			    c, b0u + BH * h, E, BH, sbc + 16 * cb + BH * h * TC,
+			    sba + arow + BH * h * 32, sbv + BH * h, esum, ecmp);
 	}
 
 	/* deterministic workgroup reduction */
