@@ -15,7 +15,9 @@
  *   epilogue -- for every (a, b, c) of the reference's triangle:
  *       disconnected term Z, denominator Dn, energy W (W - Z) / Dn
  *       ((fT): (W_L - Z_L) W_R / Dn), compensated FP64 sum per lane,
- *       one deterministic reduction per workgroup into partial[item].
+ *       one deterministic reduction per workgroup into partial[item];
+ *       the Z and Dn operands that depend on b are staged in LDS once
+ *       per item, those of a and c alone held in registers per lane.
  *
  * Operands go from HBM straight into LDS with buffer-load-to-LDS DMAs (Q
  * from the packed copies in 16-byte runs, S rows in dwords), several steps
